@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit 2D hydro step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--cpu-seconds S]
+
+Workload: the Kelvin-Helmholtz shear layer at N x N (default 4096^2), PLM-minmod / HLLC /
+SSP-RK2, CFL 0.8, periodic, viscosity mu = 1e-4 and thermal conduction kappa = 1e-4 on,
+synthetic seeded initial state, all state resident in HBM. One step = one full RK2 step
+(dt reduction + two fused stage launches). The FP64 parity form gives `value`
+(Gcell-updates/s = Nx*Ny*K / time of K steps, HIP events on the solver stream, device
+synchronised on both sides); the FP32 form is timed the same way and reported beside it.
+
+roofline: the dominant kernel is the fused stage kernel (two launches per step). Its
+algorithmic HBM bytes per cell are 160 B per FP64 RK2 step (stage 1 reads U^n and writes
+U*, stage 2 reads U* and U^n and writes U^n+1: 5 states x 4 fields x 8 B), i.e. 80 B per
+launch on average (80 B / 40 B per launch in the FP32 form, whose U is stored in FP32).
+achieved = bytes per launch x cells / mean launch time (HIP events around every stage
+launch in the timed region). traffic = measured HBM bytes per launch (rocprofv3
+FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) from the committed summary
+under profiles/, when present.
+
+cpu_baseline: the CPU oracle (a plain-C FP64 restatement of the reference path with the
+reference's pass structure, OpenMP) timed on the host cores on a bounded sample of the
+same workload (as many steps as fit in --cpu-seconds, at least one).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s & % HBM peak at 4096² FP64 vs FP32; served-job p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STEP = {"fp64": 160.0, "fp32": 80.0}
+STAGE_LAUNCHES_PER_STEP = 2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(prec: str, size: int):
+    """Measured HBM bytes per stage launch from profiles/ (None when not measured)."""
+    path = os.path.join(ROOT, "profiles", f"stage_kh{size}_{prec}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray):
+    from fv2d_amd import hiprt
+    from fv2d_amd import problems as P
+    from fv2d_amd.capi import FP32, FP64
+    from fv2d_amd.solver import Solver
+
+    p = P.kh_params(N=size, precision=FP64 if prec == "fp64" else FP32)
+    s = Solver(p)
+    s.set_prim(q0)
+    if warmup:
+        s.advance(max_steps=warmup, record_dts=False)
+    stream = s.lib.fv2d_get_stream(s.h)
+    hiprt.synchronize()
+    s.lib.fv2d_profile(s.h, 1)
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    e0.record(stream)
+    _, dts, counts, n = s.advance(max_steps=steps)
+    e1.record(stream)
+    hiprt.synchronize()
+    ms = e0.elapsed_ms(e1)
+    import ctypes as C
+
+    st_ms = C.c_double()
+    st_n = C.c_int64()
+    s.lib.fv2d_profile_read(s.h, C.byref(st_ms), C.byref(st_n))
+    s.lib.fv2d_profile(s.h, 0)
+    assert n == steps, f"advance took {n} steps, expected {steps}"
+    cells = p.Nx * p.Ny
+    gcells = cells * steps / (ms * 1e-3) / 1e9
+    mean_launch_s = st_ms.value * 1e-3 / max(st_n.value, 1)
+    bpl = BYTES_PER_CELL_STEP[prec] / STAGE_LAUNCHES_PER_STEP
+    achieved = bpl * cells / mean_launch_s / 1e9
+    q = s.get_prim()
+    sums = s.sums()
+    s.close()
+    return dict(gcells=gcells, ms_per_step=ms / steps, mean_launch_ms=mean_launch_s * 1e3, launches=int(st_n.value),
+                achieved=achieved, bytes_per_launch=bpl * cells, counts=counts, q=q, sums=sums,
+                dt_first=float(dts[0]) if len(dts) else None)
+
+
+def run_cpu(size: int, seconds: float, q0: np.ndarray):
+    from fv2d_amd import problems as P
+    from oracle import oracle as OR
+
+    threads = min(os.cpu_count() or 1, 16)
+    lib = OR.load()
+    lib.orc_set_threads(threads)
+    o = OR.Oracle(P.kh_params(N=size), lib)
+    o.set_prim(q0)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        dt, _ = o.compute_dt()
+        o.step(dt)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 50:
+            break
+    o.close()
+    return dict(value=size * size * n / el / 1e9, steps=n, seconds=el, threads=lib.orc_get_threads())
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(f"bench.py: --gpus {a.gpus} is not supported: this build measures one GPU only; the "
+              "multi-GPU decomposition is a design (y-slab halo exchange), not a measured path", file=sys.stderr)
+        return 2
+
+    from fv2d_amd import capi
+    from fv2d_amd import problems as P
+
+    capi.load_library()  # fail loudly when the HIP library is missing
+    log(f"bench: building KH {a.size}^2 initial state")
+    q0 = P.kh_ic(P.kh_params(N=a.size))
+
+    log(f"bench: FP64 form, {a.warmup} warmup + {a.steps} timed steps")
+    r64 = run_gpu("fp64", a.size, a.steps, a.warmup, q0)
+    log(f"bench: FP64 {r64['gcells']:.3f} Gcell/s, {r64['ms_per_step']:.3f} ms/step")
+    r32 = None
+    if not a.no_fp32:
+        log(f"bench: FP32 form, {a.warmup} warmup + {a.steps} timed steps")
+        r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0)
+        log(f"bench: FP32 {r32['gcells']:.3f} Gcell/s, {r32['ms_per_step']:.3f} ms/step")
+    cpu = None
+    if not a.no_cpu:
+        log(f"bench: CPU oracle baseline, ~{a.cpu_seconds:.0f} s sample")
+        cpu = run_cpu(a.size, a.cpu_seconds, q0)
+        log(f"bench: CPU {cpu['value']:.4f} Gcell/s over {cpu['steps']} steps on {cpu['threads']} threads")
+
+    cells = a.size * a.size
+    out = {
+        "metric": METRIC,
+        "value": round(r64["gcells"], 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r64["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"kelvin-helmholtz {a.size}x{a.size} fp64",
+            "grid": [a.size, a.size],
+            "reconstruction": "plm-minmod",
+            "riemann": "hllc",
+            "time_stepping": "ssp-rk2",
+            "cfl": 0.8,
+            "viscosity_mu": 1e-4,
+            "conduction_kappa": 1e-4,
+            "boundaries": "periodic",
+        },
+        "hbm_frac_step": round(BYTES_PER_CELL_STEP["fp64"] * cells / (r64["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "fused stage (2 launches per RK2 step)",
+            "achieved": round(r64["achieved"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(r64["achieved"] / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic("fp64", a.size),
+            "algorithmic_bytes_per_launch": r64["bytes_per_launch"],
+            "mean_launch_ms": round(r64["mean_launch_ms"], 4),
+        },
+    }
+    if r32 is not None:
+        q64, q32 = r64["q"], r32["q"]
+        l1 = {}
+        for k, name in enumerate(("rho", "u", "v", "prs")):
+            l1[name] = float(np.abs(q32[k] - q64[k]).mean() / max(np.abs(q64[k]).mean(), 1e-300))
+        out["fp32_form"] = {
+            "value": round(r32["gcells"], 4),
+            "ms_per_step": round(r32["ms_per_step"], 4),
+            "speedup_vs_fp64": round(r32["gcells"] / r64["gcells"], 3),
+            "hbm_frac_step": round(BYTES_PER_CELL_STEP["fp32"] * cells / (r32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "roofline": {
+                "bound": "hbm",
+                "achieved": round(r32["achieved"], 1),
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": round(r32["achieved"] / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic("fp32", a.size),
+                "algorithmic_bytes_per_launch": r32["bytes_per_launch"],
+                "mean_launch_ms": round(r32["mean_launch_ms"], 4),
+            },
+            "rel_L1_vs_fp64_after_steps": l1,
+        }
+    if cpu is not None:
+        out["cpu_baseline"] = {
+            "value": round(cpu["value"], 5),
+            "unit": "Gcell-updates/s",
+            "cores": cpu["threads"],
+            "kind": "port",
+            "sample": f"CPU oracle (FP64 restatement of the reference path, OpenMP), KH {a.size}^2, "
+                      f"{cpu['steps']} RK2 steps in {cpu['seconds']:.1f} s",
+        }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

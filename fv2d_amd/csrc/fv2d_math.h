@@ -1,0 +1,362 @@
+// fv2d_math.h -- per-cell numerics of the hydro step, shared by the HIP kernels and the
+// host emulation of the tile kernel.
+//
+// Every function keeps the reference's operation order (cited file:line), so that in
+// double precision, compiled without FMA contraction, it reproduces the reference bit for
+// bit. The real type R is double (parity form) or float (FP32 form); Ops<R> picks the
+// division and square root: IEEE-exact for double, hardware approximations for float.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define FV2D_HD __host__ __device__ inline __attribute__((always_inline))
+#else
+#define FV2D_HD inline __attribute__((always_inline))
+#endif
+
+namespace fv2d {
+
+enum : int { IR = 0, IU = 1, IV = 2, IP = 3, IE = 3 };
+
+// Option values, same as include/fv2d_amd.h (and the reference enums, SimInfo.h:41-97).
+enum : int { HLL = 0, HLLC = 1, FSLP = 2 };
+enum : int { BC_ABSORBING = 0, BC_REFLECTING = 1, BC_PERIODIC = 2 };
+enum : int { PCM = 0, PCM_WB = 1, PLM = 2 };
+enum : int { GRAV_NONE = 0, GRAV_CONSTANT = 1, GRAV_ANALYTICAL = 2 };
+enum : int { BCTC_NONE = 0, BCTC_FIXED_TEMPERATURE = 1, BCTC_FIXED_GRADIENT = 2 };
+
+template <typename R>
+struct St {
+  R v[4];
+};
+
+template <typename R>
+struct Ops;
+
+template <>
+struct Ops<double> {
+  static FV2D_HD double div(double a, double b) { return a / b; }
+  static FV2D_HD double sqrt_(double a) { return sqrt(a); }
+  static FV2D_HD double min_(double a, double b) { return fmin(a, b); }
+  static FV2D_HD double max_(double a, double b) { return fmax(a, b); }
+};
+
+template <>
+struct Ops<float> {
+  static FV2D_HD float div(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * __builtin_amdgcn_rcpf(b);
+#else
+    return a / b;
+#endif
+  }
+  static FV2D_HD float sqrt_(float a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(a);
+#else
+    return sqrtf(a);
+#endif
+  }
+  static FV2D_HD float min_(float a, float b) { return fminf(a, b); }
+  static FV2D_HD float max_(float a, float b) { return fmaxf(a, b); }
+};
+
+// Physics constants of one simulation, precomputed on the host with the reference's own
+// expressions (e.g. gm1 = gamma0 - 1.0, entho = 1.0 / (gamma0 - 1.0)), so the kernels
+// see the same bits the reference computes inline.
+struct KParams {
+  int Nx, Ny;
+  int pitch;          // elements per row of a field plane
+  int bcx, bcy;       // boundary types
+  int grav;           // gravity mode
+  int wb;             // well_balanced_flux_at_y_bc
+  int bctc_ymin, bctc_ymax;
+  int pad0;
+  long long plane;    // elements per field plane (pitch * Ny)
+  double gamma0, gm1, entho;
+  double dx, dy, inv_dx, inv_dy;  // inv_* = 1.0 / d (Viscosity.h:57-58)
+  double gxf, gyf;                // (double)(float)g: getGravity returns float (Gravity.h:39)
+  double gdx_x, gdx_y;            // gx*dx, gy*dy (Update.h:131, for FSLP)
+  double fslp_K;
+  double kappa, mu;
+  double bctc_ymin_value, bctc_ymax_value;
+  double eps_reset;               // misc.epsilon_reset_negative
+  double inv_tc, inv_visc;        // parabolic inverse dts (constant over the grid)
+  double CFL, epsilon;
+};
+
+// ------------------------------------------------------------------ state algebra
+
+// primToCons (States.h:19-30)
+template <typename R>
+FV2D_HD St<R> prim_to_cons(const St<R> &q, R gm1) {
+  typedef Ops<R> O;
+  St<R> r;
+  r.v[IR] = q.v[IR];
+  r.v[IU] = q.v[IR] * q.v[IU];
+  r.v[IV] = q.v[IR] * q.v[IV];
+  R Ek = O::div(R(0.5) * (r.v[IU] * r.v[IU] + r.v[IV] * r.v[IV]), q.v[IR]);
+  r.v[IE] = Ek + O::div(q.v[IP], gm1);
+  return r;
+}
+
+// consToPrim (States.h:32-43)
+template <typename R>
+FV2D_HD St<R> cons_to_prim(const St<R> &u, R gm1) {
+  typedef Ops<R> O;
+  St<R> r;
+  r.v[IR] = u.v[IR];
+  r.v[IU] = O::div(u.v[IU], u.v[IR]);
+  r.v[IV] = O::div(u.v[IV], u.v[IR]);
+  R Ek = R(0.5) * r.v[IR] * (r.v[IU] * r.v[IU] + r.v[IV] * r.v[IV]);
+  r.v[IP] = (u.v[IE] - Ek) * gm1;
+  return r;
+}
+
+// Negative density / pressure clamp of checkNegatives (SimInfo.h:617-626), on one cell.
+template <typename R>
+FV2D_HD void reset_negative(St<R> &q, R eps) {
+  if (q.v[IR] < R(0)) q.v[IR] = eps;
+  if (q.v[IP] < R(0)) q.v[IP] = eps;
+}
+
+// speedOfSound (States.h:45-46)
+template <typename R>
+FV2D_HD R speed_of_sound(const St<R> &q, R gamma0) {
+  return Ops<R>::sqrt_(Ops<R>::div(q.v[IP] * gamma0, q.v[IR]));
+}
+
+// swap_component (States.h:103-110) for the y sweep
+template <typename R>
+FV2D_HD St<R> swap_uv(const St<R> &q) {
+  St<R> r;
+  r.v[IR] = q.v[IR];
+  r.v[IU] = q.v[IV];
+  r.v[IV] = q.v[IU];
+  r.v[IP] = q.v[IP];
+  return r;
+}
+
+// minmod (Update.h:77-85): product sign test, then the smaller magnitude (ties -> dR).
+template <typename R>
+FV2D_HD R minmod(R dL, R dR) {
+  if (dL * dR < R(0)) return R(0);
+  return (fabs(dL) < fabs(dR)) ? dL : dR;
+}
+
+// PLM face value q + ((slope * sign) * 0.5) (Update.h:25, via the State operators
+// States.h:64-92). sign = +1 for the right face, -1 for the left face.
+template <typename R>
+FV2D_HD R plm_face(R q, R slope, R sign) {
+  return q + slope * sign * R(0.5);
+}
+
+// ------------------------------------------------------------------ Riemann solvers
+// All take primitive face states in the sweep frame (u normal) and return the flux in
+// that frame plus the interface pressure pout.
+
+// hll (RiemannSolvers.h:6-51), Davis wave-speed estimates
+template <typename R>
+FV2D_HD void hll(const St<R> &qL, const St<R> &qR, R gamma0, R gm1, St<R> &flux, R &pout) {
+  typedef Ops<R> O;
+  const R aL = speed_of_sound(qL, gamma0);
+  const R aR = speed_of_sound(qR, gamma0);
+  const R SL = O::min_(qL.v[IU] - aL, qR.v[IU] - aR);
+  const R SR = O::max_(qL.v[IU] + aL, qR.v[IU] + aR);
+
+  // physical flux (RiemannSolvers.h:21-29)
+  St<R> FL, FR;
+  {
+    const R Ek = R(0.5) * qL.v[IR] * (qL.v[IU] * qL.v[IU] + qL.v[IV] * qL.v[IV]);
+    const R E = O::div(qL.v[IP], gm1) + Ek;
+    FL.v[0] = qL.v[IR] * qL.v[IU];
+    FL.v[1] = qL.v[IR] * qL.v[IU] * qL.v[IU] + qL.v[IP];
+    FL.v[2] = qL.v[IR] * qL.v[IU] * qL.v[IV];
+    FL.v[3] = (qL.v[IP] + E) * qL.v[IU];
+  }
+  {
+    const R Ek = R(0.5) * qR.v[IR] * (qR.v[IU] * qR.v[IU] + qR.v[IV] * qR.v[IV]);
+    const R E = O::div(qR.v[IP], gm1) + Ek;
+    FR.v[0] = qR.v[IR] * qR.v[IU];
+    FR.v[1] = qR.v[IR] * qR.v[IU] * qR.v[IU] + qR.v[IP];
+    FR.v[2] = qR.v[IR] * qR.v[IU] * qR.v[IV];
+    FR.v[3] = (qR.v[IP] + E) * qR.v[IU];
+  }
+
+  if (SL >= R(0)) {
+    flux = FL;
+    pout = qL.v[IP];
+  } else if (SR <= R(0)) {
+    flux = FR;
+    pout = qR.v[IP];
+  } else {
+    // (SR*FL - SL*FR + SL*SR*(uR - uL)) / (SR - SL), uL/uR from primToCons
+    const St<R> uL = prim_to_cons(qL, gm1);
+    const St<R> uR = prim_to_cons(qR, gm1);
+    pout = R(0.5) * (qL.v[IP] + qR.v[IP]);
+    const R SLSR = SL * SR;
+    const R den = SR - SL;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) flux.v[k] = O::div(FL.v[k] * SR - FR.v[k] * SL + (uR.v[k] - uL.v[k]) * SLSR, den);
+  }
+}
+
+// hllc (RiemannSolvers.h:53-128). Only the star state of the selected branch is divided
+// out; the value of the selected branch is the reference's bit for bit.
+template <typename R>
+FV2D_HD void hllc(const St<R> &qL, const St<R> &qR, R gamma0, R entho, St<R> &flux, R &pout) {
+  typedef Ops<R> O;
+  const R rL = qL.v[IR], uL = qL.v[IU], vL = qL.v[IV], pL = qL.v[IP];
+  const R rR = qR.v[IR], uR = qR.v[IU], vR = qR.v[IV], pR = qR.v[IP];
+
+  const R ekL = R(0.5) * rL * (uL * uL + vL * vL);
+  const R EL = ekL + pL * entho;
+  const R ekR = R(0.5) * rR * (uR * uR + vR * vR);
+  const R ER = ekR + pR * entho;
+
+  const R cfastL = speed_of_sound(qL, gamma0);
+  const R cfastR = speed_of_sound(qR, gamma0);
+
+  const R cmax = O::max_(cfastL, cfastR);
+  const R SL = O::min_(uL, uR) - cmax;
+  const R SR = O::max_(uL, uR) + cmax;
+
+  const R rcL = rL * (uL - SL);
+  const R rcR = rR * (SR - uR);
+
+  const R den = rcR + rcL;
+  const R uS = O::div(rcR * uR + rcL * uL + (pL - pR), den);
+  const R pS = O::div(rcR * pL + rcL * pR + rcL * rcR * (uL - uR), den);
+
+  // Branches SL > 0, uS > 0, SR > 0, else (RiemannSolvers.h:93-122)
+  R sr, su, sv, sp, E;
+  if (SL > R(0)) {
+    sr = rL;
+    su = uL;
+    sv = vL;
+    sp = pL;
+    E = EL;
+    pout = pL;
+  } else if (SR > R(0)) {
+    // star region: left star if uS > 0, right star otherwise
+    const bool left = uS > R(0);
+    const R rX = left ? rL : rR;
+    const R uX = left ? uL : uR;
+    const R pX = left ? pL : pR;
+    const R EX = left ? EL : ER;
+    const R SX = left ? SL : SR;
+    const R dS = SX - uS;
+    sr = O::div(rX * (SX - uX), dS);
+    E = O::div((SX - uX) * EX - pX * uX + pS * uS, dS);
+    su = uS;
+    sv = left ? vL : vR;
+    sp = pS;
+    pout = pS;
+  } else if (uS > R(0)) {
+    // SR <= 0 < uS cannot happen for ordered speeds, but the reference tests uS first
+    const R dS = SL - uS;
+    sr = O::div(rL * (SL - uL), dS);
+    E = O::div((SL - uL) * EL - pL * uL + pS * uS, dS);
+    su = uS;
+    sv = vL;
+    sp = pS;
+    pout = pS;
+  } else {
+    sr = rR;
+    su = uR;
+    sv = vR;
+    sp = pR;
+    E = ER;
+    pout = pR;
+  }
+
+  flux.v[IR] = sr * su;
+  flux.v[IU] = sr * su * su + sp;
+  flux.v[IV] = flux.v[IR] * sv;
+  flux.v[IE] = (E + sp) * su;
+}
+
+// fslp (RiemannSolvers.h:137-171); Kokkos::max/min are (a < b ? b : a) / (b < a ? b : a).
+template <typename R>
+FV2D_HD void fslp(const St<R> &qL, const St<R> &qR, R gamma0, R gm1, R K, R gdx, St<R> &flux, R &pout) {
+  typedef Ops<R> O;
+  const R rhoL = qL.v[IR], uL = qL.v[IU], pL = qL.v[IP];
+  const R csL = O::sqrt_(O::div(gamma0 * pL, rhoL));
+  const R rhoR = qR.v[IR], uR = qR.v[IU], pR = qR.v[IP];
+  const R csR = O::sqrt_(O::div(gamma0 * pR, rhoR));
+
+  const R a1 = rhoL * csL, a2 = rhoR * csR;
+  const R ai = K * ((a1 < a2) ? a2 : a1);
+  const R t1 = O::div(fabs(uL), csL), t2 = O::div(fabs(uR), csR);
+  const R tm = (t1 < t2) ? t2 : t1;
+  const R theta = (tm < R(1)) ? tm : R(1);
+
+  const R ustar = R(0.5) * (uR + uL) - O::div(R(0.5), ai) * (pR - pL - R(0.5) * (rhoL + rhoR) * gdx);
+  const R Pi = R(0.5) * (pR + pL) - theta * R(0.5) * ai * (uR - uL);
+
+  const bool up = ustar > R(0);
+  const R sr = up ? qL.v[IR] : qR.v[IR];
+  const R su = up ? qL.v[IU] : qR.v[IU];
+  const R sv = up ? qL.v[IV] : qR.v[IV];
+  const R sp = up ? qL.v[IP] : qR.v[IP];
+  const R Ekstar = R(0.5) * sr * (su * su + sv * sv);
+  const R Estar = Ekstar + O::div(sp, gm1);
+
+  flux.v[IR] = ustar * sr;
+  flux.v[IU] = ustar * sr * su + Pi;
+  flux.v[IV] = ustar * sr * sv;
+  flux.v[IE] = ustar * (Estar + Pi);
+  pout = Pi;
+}
+
+template <int RIEM, typename R>
+FV2D_HD void riemann(const St<R> &qL, const St<R> &qR, R gamma0, R gm1, R entho, R K, R gdx, St<R> &flux,
+                     R &pout) {
+  if (RIEM == HLL)
+    hll(qL, qR, gamma0, gm1, flux, pout);
+  else if (RIEM == FSLP)
+    fslp(qL, qR, gamma0, gm1, K, gdx, flux, pout);
+  else
+    hllc(qL, qR, gamma0, entho, flux, pout);
+}
+
+// ------------------------------------------------------------------ boundaries
+
+// Ghost cell g (domain-relative, -2..N+1) -> source cell in [0, N) and a flag telling
+// whether the normal velocity is negated. Composes the X-then-Y fill of
+// BoundaryManager::fillBoundaries (BoundaryConditions.h:82-147): absorbing copies the
+// edge cell, reflecting mirrors 2*piv - i - 1 and negates the normal velocity, periodic
+// wraps by N. Indices beyond the ghost layers (only in partial edge tiles, never used by a
+// domain cell) are clamped into the domain.
+FV2D_HD int map_ghost(int g, int N, int bc, bool &flip) {
+  flip = false;
+  if (g >= 0 && g < N) return g;
+  int s;
+  if (bc == BC_PERIODIC) {
+    s = (g < 0) ? g + N : g - N;
+  } else if (bc == BC_REFLECTING) {
+    s = (g < 0) ? -g - 1 : 2 * N - g - 1;
+    flip = true;
+  } else {
+    s = (g < 0) ? 0 : N - 1;
+  }
+  s = (s < 0) ? 0 : ((s >= N) ? N - 1 : s);
+  return s;
+}
+
+// The hyperbolic inverse time step of one cell (ComputeDt.h:31-34).
+template <typename R>
+FV2D_HD double inv_dt_hyp(const St<R> &q, const KParams &kp) {
+  const R cs = speed_of_sound(q, R(kp.gamma0));
+  if (sizeof(R) == 8) {
+    return (double)((cs + fabs(q.v[IU])) / R(kp.dx) + (cs + fabs(q.v[IV])) / R(kp.dy));
+  } else {
+    // FP32 form: FP64 for the reduction, per-cell value from the FP32 state
+    return ((double)cs + fabs((double)q.v[IU])) * kp.inv_dx + ((double)cs + fabs((double)q.v[IV])) * kp.inv_dy;
+  }
+}
+
+}  // namespace fv2d

@@ -1,0 +1,98 @@
+// fv2d_params.h -- host-side validation of fv2d_params and the derived kernel constants.
+// Shared by the C-ABI layer and the host emulation of the kernels.
+#pragma once
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/fv2d_amd.h"
+#include "fv2d_math.h"
+
+namespace fv2d {
+
+// Reject what this build does not implement, with the reason.
+inline int check_params(const fv2d_params *p, std::string &why) {
+  char b[512];
+  if (!p) {
+    why = "params is NULL";
+    return FV2D_E_INVALID;
+  }
+  if (p->Nx < 2 || p->Ny < 2) {
+    snprintf(b, sizeof b, "Nx and Ny must be >= 2 (got %d x %d)", p->Nx, p->Ny);
+    why = b;
+    return FV2D_E_INVALID;
+  }
+  if (p->Ng < 1 || (p->reconstruction == FV2D_PLM && p->Ng < 2)) {
+    snprintf(b, sizeof b, "Nghosts=%d is too small: the stencils reach 2 cells (PLM) or 1 (PCM)", p->Ng);
+    why = b;
+    return FV2D_E_INVALID;
+  }
+  if ((long long)p->Nx * p->Ny > (1ll << 30)) {
+    why = "grid larger than 2^30 cells";
+    return FV2D_E_INVALID;
+  }
+  if (!(p->xmax > p->xmin) || !(p->ymax > p->ymin)) {
+    why = "empty domain (xmax <= xmin or ymax <= ymin)";
+    return FV2D_E_INVALID;
+  }
+  if (p->reconstruction < 0 || p->reconstruction > 2 || p->riemann_solver < 0 || p->riemann_solver > 2 ||
+      p->time_stepping < 0 || p->time_stepping > 1 || p->boundary_x < 0 || p->boundary_x > 2 || p->boundary_y < 0 ||
+      p->boundary_y > 2 || p->gravity_mode < 0 || p->gravity_mode > 2 || p->bctc_ymin < 0 || p->bctc_ymin > 2 ||
+      p->bctc_ymax < 0 || p->bctc_ymax > 2 || p->precision < 0 || p->precision > 1) {
+    why = "an option value is out of range";
+    return FV2D_E_INVALID;
+  }
+  if (p->tc_active && p->tc_mode == FV2D_TCM_B02) {
+    why = "thermal_conduction.conductivity_mode=B02 is not supported: the reference never reads its b02_* "
+          "parameters from the .ini (SimInfo.h:307-310), so its values are undefined";
+    return FV2D_E_UNSUPPORTED;
+  }
+  if (p->tc_active && p->tc_mode != FV2D_TCM_CONSTANT) {
+    why = "unknown thermal conduction mode";
+    return FV2D_E_INVALID;
+  }
+  return FV2D_OK;
+}
+
+inline void build_kparams(const fv2d_params *p, int pitch, KParams &kp) {
+  memset(&kp, 0, sizeof kp);
+  kp.Nx = p->Nx;
+  kp.Ny = p->Ny;
+  kp.pitch = pitch;
+  kp.plane = (long long)pitch * p->Ny;
+  kp.bcx = p->boundary_x;
+  kp.bcy = p->boundary_y;
+  kp.grav = p->gravity_mode;
+  kp.wb = p->well_balanced_flux_at_y_bc ? 1 : 0;
+  kp.bctc_ymin = p->tc_active ? p->bctc_ymin : 0;
+  kp.bctc_ymax = p->tc_active ? p->bctc_ymax : 0;
+  kp.gamma0 = p->gamma0;
+  kp.gm1 = p->gamma0 - 1.0;
+  kp.entho = 1.0 / (p->gamma0 - 1.0);
+  kp.dx = (p->xmax - p->xmin) / p->Nx;  // SimInfo.h:375-376
+  kp.dy = (p->ymax - p->ymin) / p->Ny;
+  kp.inv_dx = 1.0 / kp.dx;
+  kp.inv_dy = 1.0 / kp.dy;
+  kp.gxf = (double)(float)p->gx;
+  kp.gyf = (double)(float)p->gy;
+  kp.gdx_x = p->gx * kp.dx;
+  kp.gdx_y = p->gy * kp.dy;
+  kp.fslp_K = p->fslp_K;
+  kp.kappa = p->kappa;
+  kp.mu = p->mu;
+  kp.bctc_ymin_value = p->bctc_ymin_value;
+  kp.bctc_ymax_value = p->bctc_ymax_value;
+  kp.eps_reset = p->epsilon_reset_negative;
+  // ComputeDt.h:35-44: inactive terms give epsilon; constant kappa / mu give one value
+  kp.inv_tc = p->epsilon;
+  if (p->tc_active) kp.inv_tc = fmax(2.0 * p->kappa / (kp.dx * kp.dx), 2.0 * p->kappa / (kp.dy * kp.dy));
+  kp.inv_visc = p->epsilon;
+  if (p->visc_active) kp.inv_visc = fmax(2.0 * p->mu / (kp.dx * kp.dx), 2.0 * p->mu / (kp.dy * kp.dy));
+  kp.CFL = p->CFL;
+  kp.epsilon = p->epsilon;
+}
+
+}  // namespace fv2d

@@ -1,0 +1,85 @@
+"""TEST INFRASTRUCTURE: ctypes wrapper of the host emulation of the stage kernel."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from fv2d_amd.capi import Params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-j2", "-C", HERE], check=True)
+
+
+def load(sanitized: bool = False):
+    name = "libtile_emu_san.so" if sanitized else "libtile_emu.so"
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    u64p = C.POINTER(C.c_uint64)
+    lib.emu_create.restype = C.c_void_p
+    lib.emu_create.argtypes = [C.POINTER(Params), C.c_char_p]
+    lib.emu_destroy.argtypes = [C.c_void_p]
+    lib.emu_set_prim.argtypes = [C.c_void_p, dp]
+    lib.emu_compute_dt.restype = C.c_double
+    lib.emu_compute_dt.argtypes = [C.c_void_p, dp]
+    lib.emu_step.argtypes = [C.c_void_p, C.c_double, u64p]
+    lib.emu_get_prim.argtypes = [C.c_void_p, dp]
+    lib.emu_get_cons.argtypes = [C.c_void_p, dp]
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Emu:
+    def __init__(self, params: Params, lib=None):
+        self.lib = lib or load()
+        self.params = params.copy()
+        err = C.create_string_buffer(256)
+        self.h = self.lib.emu_create(C.byref(self.params), err)
+        if not self.h:
+            raise ValueError(err.value.decode())
+        self.Nx, self.Ny = params.Nx, params.Ny
+
+    def close(self):
+        if self.h:
+            self.lib.emu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_prim(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self.lib.emu_set_prim(self.h, _dp(q))
+
+    def compute_dt(self):
+        inv = np.zeros(3)
+        return self.lib.emu_compute_dt(self.h, _dp(inv)), inv
+
+    def step(self, dt):
+        c = (C.c_uint64 * 3)()
+        self.lib.emu_step(self.h, dt, c)
+        return tuple(int(x) for x in c)
+
+    def get_prim(self):
+        out = np.empty((4, self.Ny, self.Nx))
+        self.lib.emu_get_prim(self.h, _dp(out))
+        return out
+
+    def get_cons(self):
+        out = np.empty((4, self.Ny, self.Nx))
+        self.lib.emu_get_cons(self.h, _dp(out))
+        return out
