@@ -119,6 +119,32 @@ def run_cpu(size: int, seconds: float, q0: np.ndarray):
     return dict(value=size * size * n / el / 1e9, steps=n, seconds=el, threads=lib.orc_get_threads())
 
 
+def calibrate_copy(mib: int, reps: int = 4) -> int:
+    """hipMemcpy device-to-device of `mib` MiB, `reps` times: a copy kernel that reads and
+    writes exactly mib MiB per dispatch, the yardstick for the HBM byte counters."""
+    import ctypes as C
+
+    from fv2d_amd import hiprt
+
+    rt = hiprt.rt()
+    n = mib << 20
+    a_, b_ = C.c_void_p(), C.c_void_p()
+    rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    rt.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rt.hipFree.argtypes = [C.c_void_p]
+    assert rt.hipMalloc(C.byref(a_), n) == 0 and rt.hipMalloc(C.byref(b_), n) == 0
+    assert rt.hipMemset(a_, 1, n) == 0
+    hiprt.synchronize()
+    for _ in range(reps):
+        assert rt.hipMemcpy(b_, a_, n, 3) == 0  # hipMemcpyDeviceToDevice
+    hiprt.synchronize()
+    rt.hipFree(a_)
+    rt.hipFree(b_)
+    print(json.dumps({"calibrate_copy_bytes": n, "reps": reps}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +154,13 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE against a copy of known size) and exit")
     a = ap.parse_args(argv)
+
+    if a.calibrate_copy:
+        return calibrate_copy(a.calibrate_copy)
 
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported: this build measures one GPU only; the "

@@ -1,0 +1,417 @@
+"""Independent checks ("pins") of the CPU oracle.
+
+The reference ships no tests, fixtures or golden files; the only number it publishes for
+this path is the first hyperbolic dt of settings/blast.ini (README.md:86, 0.000521995).
+Everything else is pinned here by what the mathematics fixes: closed forms, exact
+invariants (free-stream preservation, symmetry, conservation), an exact Riemann solver
+(tests/exact_riemann.py), and linear-mode decay rates of the parabolic terms. Each pin
+takes the oracle library so that tests/test_planted_mistakes.py can show that every
+planted mistake in the oracle makes at least one pin fail.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+
+from fv2d_amd import problems as P
+from fv2d_amd.capi import (BC_ABSORBING, BC_PERIODIC, BC_REFLECTING, FSLP, GRAV_CONSTANT, HLL, HLLC,
+                           PCM, PLM, TS_EULER, TS_RK2, Params)
+from oracle import oracle as OR
+
+import exact_riemann as ER
+
+
+def _physical_flux(q, g):
+    r, u, v, p = q
+    E = p / (g - 1.0) + 0.5 * r * (u * u + v * v)
+    return np.array([r * u, r * u * u + p, r * u * v, (p + E) * u])
+
+
+# ------------------------------------------------------------------ point functions
+
+def pin_prim_cons_hand_values(lib):
+    # exactly representable case: U = (1, 0.5, -0.25, 0.5*(0.25+0.0625) + 2/0.5)
+    u = OR.prim_to_cons([1.0, 0.5, -0.25, 2.0], 1.5, lib)
+    assert list(u) == [1.0, 0.5, -0.25, 0.15625 + 4.0], u
+    q = OR.cons_to_prim([2.0, 1.0, -3.0, 10.0], 1.5, lib)
+    # u = 0.5, v = -1.5, Ek = 0.5*2*(0.25+2.25) = 2.5, p = (10 - 2.5)*0.5
+    assert list(q) == [2.0, 0.5, -1.5, 3.75], q
+
+
+def pin_prim_cons_roundtrip(lib):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        g = rng.uniform(1.1, 1.9)
+        q = np.array([rng.uniform(0.1, 10), rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(0.1, 10)])
+        q2 = OR.cons_to_prim(OR.prim_to_cons(q, g, lib), g, lib)
+        scale = np.array([q[0], 3.0, 3.0, q[3] + q[0] * (q[1] ** 2 + q[2] ** 2)])
+        assert np.all(np.abs(q2 - q) <= 1e-13 * scale), (q, q2)
+
+
+def pin_sound_speed(lib):
+    for q, g in [([1.0, 0, 0, 1.0], 1.4), ([0.125, 3.0, -1.0, 0.1], 1.4), ([2.0, 0, 0, 2.5], 5.0 / 3.0)]:
+        c = OR.speed_of_sound(q, g, lib)
+        assert abs(c - math.sqrt(g * q[3] / q[0])) <= 1e-15 * c
+
+
+def pin_minmod_table(lib):
+    m = lambda a, b: OR.minmod(a, b, lib)  # noqa: E731
+    assert m(1.0, 2.0) == 1.0 and m(2.0, 1.0) == 1.0
+    assert m(-1.0, -2.0) == -1.0 and m(-2.0, -1.0) == -1.0
+    assert m(1.0, -2.0) == 0.0 and m(-1.0, 2.0) == 0.0
+    assert m(0.0, 3.0) == 0.0 and m(3.0, 0.0) == 0.0
+    assert m(2.0, 2.0) == 2.0 and m(-2.0, -2.0) == -2.0
+    # equal magnitudes of equal sign return the right difference
+    assert math.copysign(1.0, m(0.0, -0.0)) == -1.0
+    # product test: an underflowing product of opposite signs counts as "same sign"
+    assert m(1e-200, -1e-200) == -1e-200
+
+
+def pin_riemann_consistency(lib):
+    rng = np.random.default_rng(2)
+    for _ in range(100):
+        g = rng.uniform(1.2, 1.8)
+        q = [rng.uniform(0.2, 5), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.2, 5)]
+        F = _physical_flux(q, g)
+        for kind in ("hll", "hllc", "fslp"):
+            f, pout = OR.riemann(kind, q, q, g, lib)
+            assert np.allclose(f, F, rtol=1e-13, atol=1e-13 * np.abs(F).max()), (kind, q, f, F)
+            assert abs(pout - q[3]) <= 1e-13 * q[3]
+
+
+def pin_riemann_supersonic(lib):
+    g = 1.4
+    qL = [1.0, 5.0, 0.3, 1.0]  # u - c > 0
+    qR = [0.5, 4.0, -0.2, 0.8]
+    for kind in ("hll", "hllc"):
+        f, pout = OR.riemann(kind, qL, qR, g, lib)
+        assert np.allclose(f, _physical_flux(qL, g), rtol=1e-15), (kind, f)
+        assert pout == qL[3]
+        m = lambda q: [q[0], -q[1], q[2], q[3]]  # noqa: E731
+        f, pout = OR.riemann(kind, m(qR), m(qL), g, lib)
+        assert np.allclose(f, _physical_flux(m(qL), g), rtol=1e-15), (kind, f)
+        assert pout == qL[3]
+
+
+def pin_hllc_stationary_contact(lib):
+    g = 1.4
+    f, pout = OR.riemann("hllc", [1.0, 0.0, 0.7, 2.0], [0.1, 0.0, -0.3, 2.0], g, lib)
+    assert f[0] == 0.0 and f[3] == 0.0 and f[2] == 0.0
+    assert abs(f[1] - 2.0) <= 4e-16 * 2 and abs(pout - 2.0) <= 4e-16 * 2
+
+
+def pin_riemann_mirror_wall(lib):
+    g = 1.4
+    rng = np.random.default_rng(3)
+    m = lambda q: [q[0], -q[1], q[2], q[3]]  # noqa: E731
+    for _ in range(50):
+        qL = [rng.uniform(0.2, 3), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.2, 3)]
+        qR = [rng.uniform(0.2, 3), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.2, 3)]
+        for kind in ("hll", "hllc"):
+            f, p1 = OR.riemann(kind, qL, qR, g, lib)
+            fm, p2 = OR.riemann(kind, m(qR), m(qL), g, lib)
+            # mirror image: mass, tangential momentum and energy fluxes change sign
+            np.testing.assert_allclose([-fm[0], fm[1], -fm[2], -fm[3]], f, rtol=1e-14, atol=1e-14)
+            assert abs(p1 - p2) <= 1e-14 * p1
+            # reflecting wall: the mirror state gives zero mass and energy flux
+            fw, _ = OR.riemann(kind, qL, m(qL), g, lib)
+            assert fw[0] == 0.0 and fw[3] == 0.0, (kind, fw)
+
+
+# Toro's tests 1-5 (Toro, ch. 4): (left rho,u,p), (right rho,u,p); HLLC / HLL flux at the
+# interface against the exact Godunov flux, loose tolerances (approximate solvers).
+TORO = [((1.0, 0.0, 1.0), (0.125, 0.0, 0.1)),
+        ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4)),
+        ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01)),
+        ((1.0, 0.0, 0.01), (1.0, 0.0, 100.0)),
+        ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950))]
+
+
+def pin_riemann_vs_exact(lib):
+    """Gross-error check of the approximate solvers: with the reference's simple wave-speed
+    estimates, HLLC / HLL fluxes stay within 20% / 45% of the exact Godunov flux on Toro's
+    tests, measured against the largest physical flux of the two initial states."""
+    g = 1.4
+    for left, right in TORO:
+        Fx = ER.godunov_flux(left, right, g)
+        qL, qR = [left[0], left[1], 0.0, left[2]], [right[0], right[1], 0.0, right[2]]
+        scale = max(np.abs(_physical_flux(qL, g)).max(), np.abs(_physical_flux(qR, g)).max(), np.abs(Fx).max())
+        for kind, tol in (("hllc", 0.2), ("hll", 0.45)):
+            f, _ = OR.riemann(kind, qL, qR, g, lib)
+            err = np.abs(f[[0, 1, 3]] - Fx) / scale
+            assert err.max() < tol, (kind, left, right, f, Fx)
+
+
+# ------------------------------------------------------------------ boundaries and dt
+
+def pin_boundaries(lib):
+    Nx, Ny = 6, 5
+    q = np.zeros((4, Ny, Nx))
+    jj, ii = np.meshgrid(np.arange(Ny), np.arange(Nx), indexing="ij")
+    for v in range(4):
+        q[v] = 1000 * v + 10 * jj + ii + 1.0  # unique, positive
+    for bx, by in itertools.product([BC_ABSORBING, BC_REFLECTING, BC_PERIODIC], repeat=2):
+        p = Params.defaults(Nx=Nx, Ny=Ny, boundary_x=bx, boundary_y=by)
+        o = OR.Oracle(p, lib)
+        o.set_prim(q)
+        Q = o.get_prim_full()
+        g = o.ibeg
+
+        def src(gi, N, bc):  # domain-relative ghost -> source, flip
+            if 0 <= gi < N:
+                return gi, False
+            if bc == BC_PERIODIC:
+                return gi % N, False
+            if bc == BC_REFLECTING:
+                return (-gi - 1 if gi < 0 else 2 * N - gi - 1), True
+            return (0 if gi < 0 else N - 1), False
+
+        for J in range(o.Nty):
+            for I in range(o.Ntx):
+                si, fx = src(I - g, Nx, bx)
+                sj, fy = src(J - g, Ny, by)
+                exp = q[:, sj, si].copy()
+                if fx:
+                    exp[1] = -exp[1]
+                if fy:
+                    exp[2] = -exp[2]
+                assert np.array_equal(Q[:, J, I], exp), (bx, by, I, J, Q[:, J, I], exp)
+        o.close()
+
+
+def pin_dt_closed_forms(lib):
+    # README.md:86: first dt_hyp of settings/blast.ini is 0.000521995 (1 / (512 sqrt(14)))
+    p = Params.defaults(Nx=256, Ny=256, gamma0=1.4, boundary_x=BC_PERIODIC, boundary_y=BC_PERIODIC, CFL=0.1)
+    o = OR.Oracle(p, lib)
+    o.set_prim(P.blast_ic(p))
+    dt, inv = o.compute_dt()
+    assert abs(1.0 / inv[0] - 0.000521995) < 5e-10, 1.0 / inv[0]
+    assert abs(dt - 0.1 / inv[0]) <= 1e-18 and inv[1] == 1e-6 and inv[2] == 1e-6
+    o.close()
+    # Sod 512x8 on [0,1]x[0,0.02]: dt0 = 0.8 / (sqrt(1.4) * (512 + 400))
+    p, q0, _ = P.workload("sod")
+    o = OR.Oracle(p, lib)
+    o.set_prim(q0)
+    dt, _ = o.compute_dt()
+    assert abs(dt - 0.8 / (math.sqrt(1.4) * 912.0)) <= 4.5e-16 * dt, dt
+    o.close()
+    # parabolic limits: 2 kappa / dx^2 and 2 mu / dy^2 with dy < dx
+    p = Params.defaults(Nx=16, Ny=32, tc_active=1, kappa=1e-3, visc_active=1, mu=2e-3)
+    o = OR.Oracle(p, lib)
+    o.set_prim(np.stack([np.ones((32, 16)), np.zeros((32, 16)), np.zeros((32, 16)), np.ones((32, 16))]))
+    _, inv = o.compute_dt()
+    assert abs(inv[1] - 2e-3 * 32 * 32) <= 1e-12 and abs(inv[2] - 4e-3 * 32 * 32) <= 1e-12, inv
+    o.close()
+
+
+def pin_gravity_float(lib):
+    p = Params.defaults(gravity_mode=GRAV_CONSTANT, gx=0.3, gy=-0.1)
+    o = OR.Oracle(p, lib)
+    assert o.gravity(3, 3, 1) == -0.10000000149011612
+    assert o.gravity(3, 3, 0) == float(np.float32(0.3))
+    o.close()
+
+
+# ------------------------------------------------------------------ whole-step invariants
+
+def _run(p, q0, nsteps, lib):
+    o = OR.Oracle(p, lib)
+    o.set_prim(q0)
+    for _ in range(nsteps):
+        dt, _ = o.compute_dt()
+        o.step(dt)
+    return o
+
+
+def pin_free_stream(lib):
+    for recon, riem, ts in itertools.product([PCM, PLM], [HLL, HLLC, FSLP], [TS_EULER, TS_RK2]):
+        p = Params.defaults(Nx=12, Ny=10, boundary_x=BC_PERIODIC, boundary_y=BC_PERIODIC, reconstruction=recon,
+                            riemann_solver=riem, time_stepping=ts, gamma0=1.4, visc_active=1, mu=1e-3,
+                            tc_active=1, kappa=1e-3)
+        q0 = np.stack([np.full((10, 12), 1.3), np.full((10, 12), 0.4), np.full((10, 12), -0.7),
+                       np.full((10, 12), 2.1)])
+        o = _run(p, q0, 5, lib)
+        q = o.get_prim()
+        for v in range(4):
+            assert np.all(q[v] == q[v][0, 0]), (recon, riem, ts, v)
+        o.close()
+
+
+def pin_y_uniform_sod(lib):
+    p, q0, _ = P.workload("sod", Nx=64, Ny=8)
+    o = _run(p, q0, 30, lib)
+    q = o.get_prim()
+    assert np.all(q == q[:, :1, :]), "y-uniform Sod lost y-uniformity"
+    assert np.all(q[2] == 0.0)
+    o.close()
+
+
+def pin_sodx_transpose_sody(lib):
+    for recon, riem in itertools.product([PCM, PLM], [HLL, HLLC]):
+        kw = dict(reconstruction=recon, riemann_solver=riem, time_stepping=TS_RK2, gamma0=1.4, CFL=0.5,
+                  boundary_x=BC_REFLECTING, boundary_y=BC_REFLECTING, visc_active=1, mu=1e-3, tc_active=1,
+                  kappa=1e-3)
+        px = Params.defaults(Nx=40, Ny=6, xmin=0.0, xmax=1.0, ymin=0.0, ymax=0.15, **kw)
+        py = Params.defaults(Nx=6, Ny=40, xmin=0.0, xmax=0.15, ymin=0.0, ymax=1.0, **kw)
+        qx = P.sod_x_ic(px)
+        qx[2] = 0.0
+        qy = np.stack([qx[0].T, qx[2].T, qx[1].T, qx[3].T])
+        ox, oy = _run(px, qx, 12, lib), _run(py, qy, 12, lib)
+        a, b = ox.get_prim(), oy.get_prim()
+        assert np.array_equal(a[0], b[0].T) and np.array_equal(a[3], b[3].T), (recon, riem)
+        assert np.array_equal(a[1], b[2].T) and np.array_equal(a[2], b[1].T), (recon, riem)
+        ox.close()
+        oy.close()
+
+
+def pin_periodic_conservation(lib):
+    p, q0, _ = P.workload("kh", N=48)
+    o = OR.Oracle(p, lib)
+    o.set_prim(q0)
+    s0 = o.sums()
+    for _ in range(25):
+        dt, _ = o.compute_dt()
+        o.step(dt)
+    s1 = o.sums()
+    scale = np.array([s0[0], np.abs(o.get_cons()[1]).sum() * o.dx * o.dy, 1.0, s0[3]])
+    assert np.all(np.abs(s1 - s0) <= 1e-13 * np.maximum(scale, 1e-3)), (s0, s1)
+
+
+def pin_reflecting_conservation(lib):
+    p, q0, _ = P.workload("sedov", N=48)
+    o = OR.Oracle(p, lib)
+    o.set_prim(q0)
+    s0 = o.sums()
+    for _ in range(40):
+        dt, _ = o.compute_dt()
+        o.step(dt)
+    s1 = o.sums()
+    assert abs(s1[0] - s0[0]) <= 1e-13 * s0[0] and abs(s1[3] - s0[3]) <= 1e-13 * s0[3], (s0, s1)
+    # the blast is symmetric: the net momentum stays zero to round-off
+    assert abs(s1[1]) < 1e-12 and abs(s1[2]) < 1e-12, s1
+
+
+def sod_l1(lib, N):
+    p, q0, run = P.workload("sod", Nx=N, Ny=4)
+    o = OR.Oracle(p, lib)
+    o.set_prim(q0)
+    t, _, _ = o.run(run["tend"])
+    q = o.get_prim()
+    x = (np.arange(N) + 0.5) / N
+    rho, _, _ = ER.profile(x, t, 0.5, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4)
+    o.close()
+    return float(np.abs(q[0, 0] - rho).mean())
+
+
+def pin_sod_convergence(lib):
+    e = [sod_l1(lib, N) for N in (100, 200, 400)]
+    assert e[0] < 0.012 and e[1] < e[0] / 1.4 and e[2] < e[1] / 1.4, e
+
+
+def _mode(N, eps):
+    return 1.0 + eps * np.sin(2.0 * math.pi * (np.arange(N) + 0.5) / N)
+
+
+def pin_tc_fourier(lib):
+    """Isobaric temperature mode: after one Euler step E changes by the discrete Laplacian."""
+    for axis in (0, 1):
+        N, eps, kappa = 32, 1e-3, 2e-3
+        p = Params.defaults(Nx=N if axis == 0 else 4, Ny=4 if axis == 0 else N, xmax=1.0 if axis == 0 else 0.125,
+                            ymax=0.125 if axis == 0 else 1.0, boundary_x=BC_PERIODIC, boundary_y=BC_PERIODIC,
+                            tc_active=1, kappa=kappa, gamma0=1.4, time_stepping=TS_EULER)
+        T = _mode(N, eps)
+        shape = (4, N) if axis == 0 else (N, 4)
+        Tf = np.broadcast_to(T, shape) if axis == 0 else np.broadcast_to(T[:, None], shape)
+        q0 = np.stack([1.0 / Tf, np.zeros(shape), np.zeros(shape), np.ones(shape)])
+        o = OR.Oracle(p, lib)
+        o.set_prim(q0)
+        E0 = o.get_cons()[3]
+        dt = 1e-3
+        o.step(dt)
+        dE = o.get_cons()[3] - E0
+        d = 1.0 / N
+        lap = (np.roll(T, -1) - 2 * T + np.roll(T, 1)) / (d * d)
+        exp = dt * kappa * lap
+        got = dE[0] if axis == 0 else dE[:, 0]
+        assert np.allclose(got, exp, rtol=1e-6, atol=1e-12), (axis, got[:4], exp[:4])
+        o.close()
+
+
+def pin_visc_shear_wave(lib):
+    """Shear wave: the viscous update is mu * dt * (second difference) / d, i.e. the face-flux
+    difference is not divided by the cell size (Viscosity.h:116)."""
+    for axis in (0, 1):
+        N, eps, mu = 32, 1e-3, 3e-3
+        p = Params.defaults(Nx=4 if axis == 0 else N, Ny=N if axis == 0 else 4, xmax=0.125 if axis == 0 else 1.0,
+                            ymax=1.0 if axis == 0 else 0.125, boundary_x=BC_PERIODIC, boundary_y=BC_PERIODIC,
+                            visc_active=1, mu=mu, gamma0=1.4, time_stepping=TS_EULER)
+        w = _mode(N, eps) - 1.0
+        if axis == 0:  # u(y)
+            shape = (N, 4)
+            q0 = np.stack([np.ones(shape), np.broadcast_to(w[:, None], shape), np.zeros(shape), np.ones(shape)])
+        else:  # v(x)
+            shape = (4, N)
+            q0 = np.stack([np.ones(shape), np.zeros(shape), np.broadcast_to(w, shape), np.ones(shape)])
+        o = OR.Oracle(p, lib)
+        o.set_prim(q0)
+        U0 = o.get_cons()
+        dt = 1e-3
+        o.step(dt)
+        dU = o.get_cons() - U0
+        d = 1.0 / N
+        exp = dt * mu * (np.roll(w, -1) - 2 * w + np.roll(w, 1)) / d
+        got = dU[1][:, 0] if axis == 0 else dU[2][0]
+        assert np.allclose(got, exp, rtol=1e-6, atol=1e-13), (axis, got[:4], exp[:4])
+        o.close()
+
+
+def pin_gravity_energy(lib):
+    """Closed box, constant gravity: the energy source dt * 0.5 * (F_L + F_R) * g
+    (Update.h:165) sums the same face mass fluxes that move the potential energy, so
+    sum(E) + sum(rho * phi), phi = -g y, is conserved to round-off."""
+    for riem, wb in ((HLLC, 0), (HLL, 1)):
+        p, q0, _ = P.workload("rt", Nx=24, Ny=48, riemann_solver=riem, well_balanced_flux_at_y_bc=wb)
+        o = OR.Oracle(p, lib)
+        o.set_prim(q0)
+        y = p.ymin + (np.arange(p.Ny) + 0.5) * o.dy
+        g = o.gravity(0, 0, 1)
+
+        def total():
+            u = o.get_cons()
+            return (u[3].sum() + (u[0] * (-g) * y[:, None]).sum()) * o.dx * o.dy
+
+        e0 = total()
+        for _ in range(30):
+            dt, _ = o.compute_dt()
+            o.step(dt)
+        e1 = total()
+        assert abs(e1 - e0) <= 1e-12 * abs(e0), (riem, wb, e0, e1)
+        o.close()
+
+
+def pin_check_negatives(lib):
+    p = Params.defaults(Nx=8, Ny=4, epsilon_reset_negative=1e-8)
+    o = OR.Oracle(p, lib)
+    q = np.ones((4, 4, 8))
+    o.set_prim(q)
+    q[0, 1, 2] = -1.0
+    q[3, 2, 5] = -2.0
+    q[3, 0, 0] = -3.0
+    q[1, 3, 7] = np.nan
+    o.poke_prim(q)
+    c = o.check_negatives()
+    assert c == (1, 2, 1), c
+    g = o.get_prim()
+    assert g[0, 1, 2] == 1e-8 and g[3, 2, 5] == 1e-8 and g[3, 0, 0] == 1e-8 and np.isnan(g[1, 3, 7])
+    o.close()
+
+
+ALL_PINS = [
+    pin_prim_cons_hand_values, pin_prim_cons_roundtrip, pin_sound_speed, pin_minmod_table,
+    pin_riemann_consistency, pin_riemann_supersonic, pin_hllc_stationary_contact, pin_riemann_mirror_wall,
+    pin_riemann_vs_exact, pin_boundaries, pin_dt_closed_forms, pin_gravity_float, pin_free_stream,
+    pin_y_uniform_sod, pin_sodx_transpose_sody, pin_periodic_conservation, pin_reflecting_conservation,
+    pin_sod_convergence, pin_tc_fourier, pin_visc_shear_wave, pin_gravity_energy, pin_check_negatives,
+]

@@ -1,0 +1,74 @@
+"""C-ABI library checks that need no GPU: it loads, exports every function include/*.h
+declares, its defaults are the reference's .ini defaults, rejected options fail with the
+right code and message, and a create without a GPU fails cleanly."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from fv2d_amd import capi
+from fv2d_amd.capi import Params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names += re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(fv2d_\w+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.load_library()
+    declared = header_functions()
+    assert len(declared) >= 17
+    for name in declared:
+        assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+    assert sorted(declared) == sorted(capi.EXPORTED_SYMBOLS)
+
+
+def test_abi_version_and_defaults():
+    lib = capi.load_library()
+    assert lib.fv2d_abi_version() == 1
+    p = Params()
+    lib.fv2d_params_default(C.byref(p))
+    assert p.as_dict() == Params.defaults().as_dict()
+
+
+@pytest.mark.parametrize("kw,code,frag", [
+    (dict(tc_active=1, tc_mode=1), capi.FV2D_E_UNSUPPORTED, "B02"),
+    (dict(Nx=1), capi.FV2D_E_INVALID, "Nx and Ny"),
+    (dict(reconstruction=2, Ng=1), capi.FV2D_E_INVALID, "Nghosts"),
+    (dict(xmax=-1.0), capi.FV2D_E_INVALID, "empty domain"),
+    (dict(riemann_solver=7), capi.FV2D_E_INVALID, "out of range"),
+])
+def test_rejected_options(kw, code, frag):
+    lib = capi.load_library()
+    p = Params.defaults(**kw)
+    assert lib.fv2d_params_check(C.byref(p)) == code
+    assert frag in lib.fv2d_last_error(None).decode()
+    h = C.c_void_p()
+    assert lib.fv2d_create(C.byref(p), C.byref(h)) == code and not h.value
+
+
+def test_accepted_options():
+    lib = capi.load_library()
+    for kw in (dict(), dict(reconstruction=1, riemann_solver=2, gravity_mode=2, tc_active=1, bctc_ymin=1,
+                            visc_active=1, time_stepping=1, precision=1)):
+        assert lib.fv2d_params_check(C.byref(Params.defaults(**kw))) == 0
+
+
+def test_create_without_gpu_fails_cleanly():
+    from fv2d_amd import hiprt
+    if hiprt.device_count() > 0:
+        pytest.skip("a GPU is present")
+    lib = capi.load_library()
+    h = C.c_void_p()
+    rc = lib.fv2d_create(C.byref(Params.defaults()), C.byref(h))
+    assert rc in (capi.FV2D_E_HIP, capi.FV2D_E_NOMEM) and not h.value
+    assert lib.fv2d_last_error(None)

@@ -1,0 +1,105 @@
+"""The stage kernel's phase code (fv2d_amd/csrc/fv2d_stage.h), run by the host emulation
+(tests/emu), against the CPU oracle: bitwise-equal dt, reset counts and fields for every
+combination of boundary types, reconstruction, Riemann solver, time stepping, conduction,
+viscosity, gravity and the well-balanced wall flux, on an odd grid (partial tiles). The
+sanitized build (ASan + UBSan, LDS image poisoned with NaN) checks every global and LDS
+index of the same code."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import emu as EMU
+from fv2d_amd import problems as P
+from fv2d_amd.capi import FP32, PLM, Params
+from oracle.oracle import Oracle
+
+COMBOS = list(itertools.product([0, 1, 2], [0, 1, 2], [0, 1, 2], [0, 1, 2], [0, 1], [0, 1], [0, 1], [0, 1, 2], [0, 1]))
+
+
+def make(n, bcx, bcy, recon, riem, ts, tc, vi, grav, wb, Nx=37, Ny=19):
+    p = Params.defaults(Nx=Nx, Ny=Ny, xmin=0.1, xmax=1.3, ymin=-0.2, ymax=0.7, boundary_x=bcx, boundary_y=bcy,
+                        reconstruction=recon, riemann_solver=riem, time_stepping=ts, tc_active=tc, kappa=3e-3,
+                        visc_active=vi, mu=2e-3, gravity_mode=grav, gx=0.3, gy=-0.7, hot_bubble_g0=0.5,
+                        well_balanced_flux_at_y_bc=wb, gamma0=1.4, CFL=0.4,
+                        bctc_ymin=1 if (tc and bcy == 1) else 0, bctc_ymax=2 if (tc and bcy == 1) else 0,
+                        bctc_ymin_value=1.5, bctc_ymax_value=0.3)
+    return p, P.random_state(p, seed=n)
+
+
+def compare(p, q0, lib, oracle_lib, steps=3):
+    o, e = Oracle(p, oracle_lib), EMU.Emu(p, lib)
+    o.set_prim(q0)
+    e.set_prim(q0)
+    for s in range(steps):
+        dto, _ = o.compute_dt()
+        dte, _ = e.compute_dt()
+        assert dte == dto, f"step {s}: dt {dte!r} != {dto!r}"
+        co, ce = o.step(dto), e.step(dte)
+        assert ce == co
+        qo, qe = o.get_prim(), e.get_prim()
+        assert np.array_equal(qo, qe, equal_nan=True), f"step {s}: max diff {np.nanmax(np.abs(qo - qe))}"
+
+
+@pytest.fixture(scope="module")
+def emu_lib():
+    EMU.build()
+    return EMU.load()
+
+
+def test_all_switch_combinations_bitwise(emu_lib, oracle_lib):
+    for n, c in enumerate(COMBOS):
+        p, q0 = make(n, *c)
+        compare(p, q0, emu_lib, oracle_lib)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(2, 2), (31, 9), (32, 8), (33, 7), (64, 3), (5, 70)])
+def test_grid_shapes_bitwise(Nx, Ny, emu_lib, oracle_lib):
+    for n, c in enumerate([(1, 1, PLM, 1, 1, 1, 1, 1, 1), (2, 0, PLM, 0, 0, 0, 1, 0, 0), (0, 2, 0, 2, 1, 1, 0, 2, 1)]):
+        p, q0 = make(n, *c, Nx=Nx, Ny=Ny)
+        compare(p, q0, emu_lib, oracle_lib, steps=2)
+
+
+def test_sanitized_bounds(oracle_lib):
+    """ASan/UBSan build: every 9th combination, 2 steps; a bad index aborts the process, so
+    this test runs the sanitized library in a child interpreter."""
+    import subprocess
+    import sys
+    EMU.build()
+    code = (
+        "import sys; sys.path[:0] = %r\n"
+        "import test_emulated_kernel as T, emu as EMU\n"
+        "from oracle import oracle as OR\n"
+        "lib = EMU.load(sanitized=True); olib = OR.load()\n"
+        "for n, c in enumerate(T.COMBOS):\n"
+        "    if n %% 9: continue\n"
+        "    p, q0 = T.make(n, *c)\n"
+        "    T.compare(p, q0, lib, olib, steps=2)\n"
+        "print('ok')\n" % ([os.path.dirname(__file__), os.path.dirname(os.path.dirname(__file__)),
+                           os.path.join(os.path.dirname(__file__), "emu")],))
+    import subprocess as sp
+    asan = sp.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+    ubsan = sp.run(["g++", "-print-file-name=libubsan.so"], capture_output=True, text=True).stdout.strip()
+    env = dict(os.environ, LD_PRELOAD=f"{asan}:{ubsan}", ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=1200)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stderr[-3000:]
+
+
+def test_fp32_form_close_to_fp64(emu_lib, oracle_lib):
+    """FP32 form (emulated): KH 64^2, 20 steps, relative L1 of every field <= 1e-5."""
+    p, q0, _ = P.workload("kh", N=64)
+    o = Oracle(p, oracle_lib)
+    e = EMU.Emu(p.copy(precision=FP32), emu_lib)
+    o.set_prim(q0)
+    e.set_prim(q0)
+    for _ in range(20):
+        dto, _ = o.compute_dt()
+        dte, _ = e.compute_dt()
+        assert abs(dte - dto) <= 1e-5 * dto
+        o.step(dto)
+        e.step(dte)
+    qo, qe = o.get_prim(), e.get_prim()
+    for k in range(4):
+        l1 = np.abs(qe[k] - qo[k]).mean() / np.abs(qo[k]).mean()
+        assert l1 <= 1e-5, (k, l1)
