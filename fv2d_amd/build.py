@@ -39,10 +39,10 @@ UNITS = [
 ]
 
 
-def _compile(unit):
+def _compile(unit, defines=(), objdir=OBJ):
     src, extra = unit
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, *COMMON, *extra, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass does not know the packed-fp32 feature and says so; that is harmless
     noise = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
@@ -51,30 +51,34 @@ def _compile(unit):
     return obj, noise
 
 
-def build(jobs: int = 3, verbose: bool = False) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(jobs: int = 3, verbose: bool = False, defines=(), out: str = LIB) -> str:
+    """Compile every unit and link `out`. `defines` (NAME=VALUE) select build variants."""
+    objdir = OBJ if not defines else OBJ + "_" + "_".join(d.replace("=", "") for d in defines)
+    os.makedirs(objdir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        results = list(ex.map(_compile, UNITS))
+        results = list(ex.map(lambda u: _compile(u, defines, objdir), UNITS))
     objs = [o for o, _ in results]
     if verbose:
         for _, noise in results:
             for ln in noise:
                 print(ln, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=3)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--define", action="append", default=[], help="NAME=VALUE for a build variant")
+    ap.add_argument("--out", default=LIB)
     a = ap.parse_args(argv)
-    print(build(a.jobs, a.verbose))
+    print(build(a.jobs, a.verbose, tuple(a.define), a.out))
 
 
 if __name__ == "__main__":

@@ -66,7 +66,8 @@ class Params(C.Structure):
         ("epsilon", C.c_double),
         ("epsilon_reset_negative", C.c_double),
         ("precision", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("seg_rows", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
     @classmethod
@@ -100,6 +101,7 @@ class Params(C.Structure):
         p.epsilon = 1.0e-6
         p.epsilon_reset_negative = 1.0e-8
         p.precision = FP64
+        p.seg_rows = 0
         for k, v in kw.items():
             if not hasattr(p, k):
                 raise AttributeError(f"fv2d_params has no field {k!r}")

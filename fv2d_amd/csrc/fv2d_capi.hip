@@ -31,6 +31,7 @@ struct fv2d_handle {
   StepCtl *ctl = nullptr;
   double *partials = nullptr;
   int nparts = 0;
+  StripGeom geom;
   float *grav_row = nullptr;
   double *staging = nullptr;  // 4 * Nx * Ny doubles
   double *rowsum = nullptr;   // 4 * Ny doubles
@@ -86,18 +87,19 @@ __global__ void init_state_kernel(KParams kp, const double *q, S *U, S *Q0) {
   }
 }
 
-// Max hyperbolic inverse dt per stage-tile workgroup (same partial layout as the stage
-// epilogue), over Q0 or over reset(consToPrim(U)).
+// Max hyperbolic inverse dt over Q0 or over reset(consToPrim(U)): workgroup b covers a
+// contiguous run of cells and writes partials[b] (the stage epilogue's partial layout).
 template <typename R, typename S>
-__global__ void __launch_bounds__(STAGE_TX *STAGE_TY) dt_tiles_kernel(KParams kp, const S *src, int from_q0,
-                                                                      double *partials) {
-  const int xl = threadIdx.x % STAGE_TX, yl = threadIdx.x / STAGE_TX;
-  const int gi = blockIdx.x * STAGE_TX + xl, gj = blockIdx.y * STAGE_TY + yl;
-  CellRegs r;
-  r.neg_rho = r.neg_prs = r.nan_cnt = 0;
-  r.inv = 0.0;
-  if (gi < kp.Nx && gj < kp.Ny) {
-    const long long k = (long long)gj * kp.pitch + gi;
+__global__ void __launch_bounds__(256) dt_cells_kernel(KParams kp, const S *src, int from_q0, double *partials,
+                                                       int nparts) {
+  const long long n = (long long)kp.Nx * kp.Ny;
+  const long long per = (n + nparts - 1) / nparts;
+  const long long c0 = blockIdx.x * per;
+  const long long c1 = (c0 + per < n) ? c0 + per : n;
+  double m = 0.0;
+  for (long long c = c0 + threadIdx.x; c < c1; c += 256) {
+    const int j = (int)(c / kp.Nx), i = (int)(c % kp.Nx);
+    const long long k = (long long)j * kp.pitch + i;
     St<R> q;
 #pragma unroll
     for (int v = 0; v < 4; ++v) q.v[v] = R(src[v * kp.plane + k]);
@@ -106,17 +108,13 @@ __global__ void __launch_bounds__(STAGE_TX *STAGE_TY) dt_tiles_kernel(KParams kp
       reset_negative(q, R(kp.eps_reset));
     }
     const double inv = inv_dt_hyp(q, kp);
-    r.inv = (inv == inv) ? inv : 0.0;
+    if (inv > m) m = inv;
   }
-  __shared__ double red[STAGE_TX * STAGE_TY / 64];
-  const double m = wave_max(r.inv);
+  __shared__ double red[4];
+  m = wave_max(m);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = red[0];
-    for (int i = 1; i < STAGE_TX * STAGE_TY / 64; ++i) b = fmax(b, red[i]);
-    partials[blockIdx.x + blockIdx.y * gridDim.x] = b;
-  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
 // Reduce the partial maxima into ctl->inv_hyp; in advance mode also run the loop control
@@ -252,13 +250,13 @@ static void set_global_error(const char *msg) { g_err = msg; }
 static bool is_fp32(const fv2d_handle *h) { return h->p.precision == FV2D_FP32; }
 
 static int launch_dt_tiles(fv2d_handle *h, const void *src, int from_q0) {
-  const dim3 grid(stage_grid_x(h->p.Nx), stage_grid_y(h->p.Ny));
+  const dim3 grid(h->nparts);
   if (is_fp32(h))
-    hipLaunchKernelGGL((dt_tiles_kernel<float, float>), grid, dim3(STAGE_TX * STAGE_TY), 0, h->stream, h->kp,
-                       (const float *)src, from_q0, h->partials);
+    hipLaunchKernelGGL((dt_cells_kernel<float, float>), grid, dim3(256), 0, h->stream, h->kp, (const float *)src,
+                       from_q0, h->partials, h->nparts);
   else
-    hipLaunchKernelGGL((dt_tiles_kernel<double, double>), grid, dim3(STAGE_TX * STAGE_TY), 0, h->stream, h->kp,
-                       (const double *)src, from_q0, h->partials);
+    hipLaunchKernelGGL((dt_cells_kernel<double, double>), grid, dim3(256), 0, h->stream, h->kp, (const double *)src,
+                       from_q0, h->partials, h->nparts);
   HIPH(h, hipGetLastError());
   return FV2D_OK;
 }
@@ -297,8 +295,8 @@ static int launch_one_stage(fv2d_handle *h, const void *src, int qsrc, const voi
     io.partials = h->partials;
     io.qsrc = qsrc;
     io.epilogue = epilogue;
-    e = launch_stage_fp32(h->kp, io, recon, h->p.riemann_solver, h->p.tc_active != 0, h->p.visc_active != 0,
-                          h->stream);
+    e = launch_stage_fp32(h->kp, io, h->geom, recon, h->p.riemann_solver, h->p.tc_active != 0,
+                          h->p.visc_active != 0, h->stream);
   } else {
     StageIO<double> io;
     io.src = (const double *)src;
@@ -310,8 +308,8 @@ static int launch_one_stage(fv2d_handle *h, const void *src, int qsrc, const voi
     io.partials = h->partials;
     io.qsrc = qsrc;
     io.epilogue = epilogue;
-    e = launch_stage_fp64(h->kp, io, recon, h->p.riemann_solver, h->p.tc_active != 0, h->p.visc_active != 0,
-                          h->stream);
+    e = launch_stage_fp64(h->kp, io, h->geom, recon, h->p.riemann_solver, h->p.tc_active != 0,
+                          h->p.visc_active != 0, h->stream);
   }
   if (e != hipSuccess) FAILH(h, FV2D_E_HIP, "stage launch: %s", hipGetErrorString(e));
   if (h->prof && (rc = prof_mark(h))) return rc;
@@ -379,6 +377,7 @@ void fv2d_params_default(fv2d_params *p) {
   p->epsilon = 1.0e-6;
   p->epsilon_reset_negative = 1.0e-8;
   p->precision = FV2D_FP64;
+  p->seg_rows = 0;
 }
 
 int fv2d_params_check(const fv2d_params *p) {
@@ -425,7 +424,8 @@ int fv2d_create(const fv2d_params *p, fv2d_handle **out) {
   h->esize = (p->precision == FV2D_FP32) ? 4 : 8;
   const int pitch = (p->Nx + 63) / 64 * 64;
   build_kparams(p, pitch, h->kp);
-  h->nparts = stage_grid_x(p->Nx) * stage_grid_y(p->Ny);
+  h->geom = strip_geom(p->Nx, p->Ny, pick_seg_rows(p));
+  h->nparts = h->geom.nblocks;
 
   auto fail = [&](const char *what, hipError_t e) {
     char b[512];

@@ -36,9 +36,30 @@ struct St {
 template <typename R>
 struct Ops;
 
+// a / d for a divisor d fixed over the grid (dx or dy), given rd = RN(1/d).
+// Markstein's correction (P. Markstein, IBM J. Res. Dev. 34(1), 1990): with q0 = RN(a*rd)
+// within one ulp of a/d, r = a - d*q0 is exact in one FMA and RN(q0 + r*rd) is the
+// correctly rounded quotient, barring underflow and overflow. The fast path is taken only
+// for 2^-960 <= |a| <= 2^960 and when the host checked 2^-60 <= d <= 2^60 (fast != 0), so
+// every intermediate stays normal; zeros return a itself (a / d keeps the sign of a);
+// anything else (tiny, huge, inf, NaN) uses the IEEE division.
+FV2D_HD double markstein_div(double a, double d, double rd, int fast) {
+  const double aa = fabs(a);
+  if (fast) {
+    if (aa >= 0x1p-960 && aa <= 0x1p+960) {
+      const double q0 = a * rd;
+      const double r = fma(-d, q0, a);
+      return fma(r, rd, q0);
+    }
+    if (aa == 0.0) return a;
+  }
+  return a / d;
+}
+
 template <>
 struct Ops<double> {
   static FV2D_HD double div(double a, double b) { return a / b; }
+  static FV2D_HD double div_c(double a, double d, double rd, int fast) { return markstein_div(a, d, rd, fast); }
   static FV2D_HD double sqrt_(double a) { return sqrt(a); }
   static FV2D_HD double min_(double a, double b) { return fmin(a, b); }
   static FV2D_HD double max_(double a, double b) { return fmax(a, b); }
@@ -52,6 +73,11 @@ struct Ops<float> {
 #else
     return a / b;
 #endif
+  }
+  static FV2D_HD float div_c(float a, float d, float rd, int fast) {
+    (void)d;
+    (void)fast;
+    return a * rd;
   }
   static FV2D_HD float sqrt_(float a) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -74,10 +100,10 @@ struct KParams {
   int grav;           // gravity mode
   int wb;             // well_balanced_flux_at_y_bc
   int bctc_ymin, bctc_ymax;
-  int pad0;
+  int fastdiv;        // dx and dy allow the Markstein division (markstein_div)
   long long plane;    // elements per field plane (pitch * Ny)
   double gamma0, gm1, entho;
-  double dx, dy, inv_dx, inv_dy;  // inv_* = 1.0 / d (Viscosity.h:57-58)
+  double dx, dy, inv_dx, inv_dy;  // inv_* = 1.0 / d (Viscosity.h:57-58), also RN(1/d) for markstein_div
   double gxf, gyf;                // (double)(float)g: getGravity returns float (Gravity.h:39)
   double gdx_x, gdx_y;            // gx*dx, gy*dy (Update.h:131, for FSLP)
   double fslp_K;
@@ -352,7 +378,8 @@ template <typename R>
 FV2D_HD double inv_dt_hyp(const St<R> &q, const KParams &kp) {
   const R cs = speed_of_sound(q, R(kp.gamma0));
   if (sizeof(R) == 8) {
-    return (double)((cs + fabs(q.v[IU])) / R(kp.dx) + (cs + fabs(q.v[IV])) / R(kp.dy));
+    return markstein_div((double)(cs + fabs(q.v[IU])), kp.dx, kp.inv_dx, kp.fastdiv) +
+           markstein_div((double)(cs + fabs(q.v[IV])), kp.dy, kp.inv_dy, kp.fastdiv);
   } else {
     // FP32 form: FP64 for the reduction, per-cell value from the FP32 state
     return ((double)cs + fabs((double)q.v[IU])) * kp.inv_dx + ((double)cs + fabs((double)q.v[IV])) * kp.inv_dy;

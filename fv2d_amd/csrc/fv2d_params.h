@@ -45,6 +45,11 @@ inline int check_params(const fv2d_params *p, std::string &why) {
     why = "an option value is out of range";
     return FV2D_E_INVALID;
   }
+  if (p->seg_rows < 0 || p->seg_rows % 4 != 0) {
+    snprintf(b, sizeof b, "seg_rows=%d must be 0 or a positive multiple of 4", p->seg_rows);
+    why = b;
+    return FV2D_E_INVALID;
+  }
   if (p->tc_active && p->tc_mode == FV2D_TCM_B02) {
     why = "thermal_conduction.conductivity_mode=B02 is not supported: the reference never reads its b02_* "
           "parameters from the .ini (SimInfo.h:307-310), so its values are undefined";
@@ -76,6 +81,7 @@ inline void build_kparams(const fv2d_params *p, int pitch, KParams &kp) {
   kp.dy = (p->ymax - p->ymin) / p->Ny;
   kp.inv_dx = 1.0 / kp.dx;
   kp.inv_dy = 1.0 / kp.dy;
+  kp.fastdiv = (kp.dx >= 0x1p-60 && kp.dx <= 0x1p+60 && kp.dy >= 0x1p-60 && kp.dy <= 0x1p+60) ? 1 : 0;
   kp.gxf = (double)(float)p->gx;
   kp.gyf = (double)(float)p->gy;
   kp.gdx_x = p->gx * kp.dx;
@@ -93,6 +99,17 @@ inline void build_kparams(const fv2d_params *p, int pitch, KParams &kp) {
   if (p->visc_active) kp.inv_visc = fmax(2.0 * p->mu / (kp.dx * kp.dx), 2.0 * p->mu / (kp.dy * kp.dy));
   kp.CFL = p->CFL;
   kp.epsilon = p->epsilon;
+}
+
+// Rows per workgroup segment: 64 by default (a 4-row prologue per segment, enough
+// workgroups to fill 256 CUs at 4096^2), fewer on short grids so that small grids still
+// spread over the chip.
+inline int pick_seg_rows(const fv2d_params *p) {
+  if (p->seg_rows > 0) return p->seg_rows;
+  int seg = 64;
+  const long long strips = (p->Nx + 61) / 62;
+  while (seg > 8 && strips * ((p->Ny + seg - 1) / seg) < 2048) seg -= 8;
+  return seg;
 }
 
 }  // namespace fv2d

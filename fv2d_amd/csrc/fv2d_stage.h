@@ -1,24 +1,30 @@
-// fv2d_stage.h -- the fused stage of the hydro step, tiled for LDS.
+// fv2d_stage.h -- the fused stage of the hydro step: column-strip marching, one wave per
+// row, one lane per column.
 //
-// One workgroup owns a TX x TY tile of domain cells, one cell per thread. A stage is
-// three barrier-separated phases:
-//   load   : the primitives of the tile plus a 2-cell halo go to LDS. They are computed on
-//            the fly from U (consToPrim + negative reset) or taken from the initial Q, and
-//            ghost cells are synthesised from the boundary rules (map_ghost), so no ghost
-//            is ever stored in HBM. Each thread also keeps its own cell's U in registers.
-//   flux   : every x and y interface of the tile is solved once (PLM-minmod slopes and
-//            face states from LDS, then HLL / HLLC / FSLP); fluxes go to LDS.
-//   update : each thread applies, in the reference's accumulation order, the x flux
-//            difference (+ x gravity), the y flux difference (+ y gravity and the
-//            well-balanced wall flux), thermal conduction and viscosity to its U; RK2
-//            stage 2 averages with U^n; the result is stored. The optional epilogue
-//            converts the new U to primitives, counts negative / NaN values and forms the
-//            cell's hyperbolic inverse dt for the next step's reduction.
-// Interface fluxes are computed once per interface, whereas the reference solves each
-// interface twice with the same arguments (Update.h:142-143): same bits, half the work.
+// A workgroup of 4 wave64s owns a strip of 62 domain columns and a segment of rows, and
+// marches up the segment 4 rows (one "chunk") at a time. Lane l of every wave stands for
+// strip column l (columns 0..65 of the strip are domain columns i0-2 .. i0+63; 2..63 are
+// the 62 output columns), wave w for row jc+w of the current chunk. Per chunk:
+//   load   wave w turns the U of row jc+2+w into primitives (consToPrim + negative
+//          reset, or the initial Q) and stores them, and T = p/rho, in a 7-row LDS ring.
+//          Ghost cells are synthesised from the boundary rules (map_ghost); no ghost is
+//          stored in HBM.
+//   pass 1 x slope of column l+1 (minmod), shifted one lane right (DPP) to get column l's;
+//          the x interface (l, l+1) is solved in registers. The y slope of row jc+w+1 is
+//          computed once and put in an LDS ring for the wave above.
+//   pass 2 the y interface (jc+w, jc+w+1) is solved with the two y slopes; its flux stays
+//          in registers (top face of this wave's cell) and goes to an LDS ring (bottom face
+//          of the next row's cell).
+//   update the x-face fluxes of the left neighbour come from the lane to the left (DPP);
+//          the hydro, gravity, conduction and viscosity terms are applied in the reference's
+//          accumulation order; RK2 stage 2 averages with U^n. The optional epilogue forms
+//          the next step's primitives (reset counts) and hyperbolic inverse dt.
+// Every interface flux, slope and x-face conduction / viscous flux is computed once;
+// the reference solves each interface twice with the same arguments (Update.h:142-143)
+// and evaluates the same face terms from both sides, with the same bits.
 //
-// The phase functions are plain host/device code; the HIP kernel calls them with
-// __syncthreads() between phases, the host emulation calls them thread by thread.
+// The phase functions are plain host/device code. The HIP kernel runs them with barriers
+// and DPP lane shifts in between; the host emulation runs them lane by lane.
 #pragma once
 
 #include "fv2d_math.h"
@@ -52,35 +58,74 @@ struct StageIO {
   int epilogue;
 };
 
-// Per-thread state carried between phases, and the epilogue results.
-struct CellRegs {
-  double u[4];
-  double u0[4];
+// Strip / segment geometry shared by the launcher, the kernel and the emulation.
+struct StripGeom {
+  int nsx, nsy;  // strips across x, segments along y
+  int seg;       // rows per segment (multiple of 4)
+  int nblocks;
+};
+
+constexpr int STRIP_OUT = 62;  // output columns per strip
+constexpr int STRIP_W = 64;    // lanes = strip columns held in registers
+constexpr int STRIP_ROWS = 4;  // rows per chunk = waves per workgroup
+
+FV2D_HD StripGeom strip_geom(int Nx, int Ny, int seg) {
+  StripGeom g;
+  g.seg = seg;
+  g.nsx = (Nx + STRIP_OUT - 1) / STRIP_OUT;
+  g.nsy = (Ny + seg - 1) / seg;
+  g.nblocks = g.nsx * g.nsy;
+  return g;
+}
+
+// Workgroup id -> (strip, segment). Consecutive workgroup ids are dispatched round-robin
+// over the 8 XCDs; when the count allows, XCD x gets a contiguous run of strips so that
+// x-neighbouring strips (which share halo columns) run on the same L2.
+FV2D_HD void strip_of_block(const StripGeom &g, int blk, int &sx, int &sy) {
+  int logical = blk;
+  if (g.nblocks % 8 == 0) logical = (blk % 8) * (g.nblocks / 8) + blk / 8;
+  sx = logical % g.nsx;
+  sy = logical / g.nsx;
+}
+
+// Per-lane state carried between phases (registers on the GPU).
+template <typename R>
+struct LaneRegs {
+  double u[4];   // own cell U (base of the update)
+  double u0[4];  // own cell U^n (RK2 average)
+  R sx[4];       // x slope of strip column l+1
+  R sxl[4];      // x slope of column l (from lane l-1)
+  R syu[4];      // y slope of (l, row+1)
+  R fx[4];       // flux at x interface (l, l+1)
+  R fxl[4];      // flux at x interface (l-1, l) (from lane l-1)
+  R fy[4];       // flux at the top y interface of the cell
+  R py;          // its interface pressure
+  R tcx, tcxl;   // conduction flux at x faces (l, l+1) and (l-1, l)
+  R vx[3], vxl[3];  // viscous x-face terms mu*tau_xx, mu*tau_xy, mu*(tau.u) at (l,l+1), (l-1,l)
   unsigned neg_rho, neg_prs, nan_cnt;
   double inv;
 };
 
-template <typename R, typename S, int TX_, int TY_, int RECON, int RIEM, bool TC, bool VISC>
-struct TileStage {
-  static constexpr int TX = TX_, TY = TY_, NT = TX_ * TY_;
-  static constexpr int HX = TX + 4, HY = TY + 4;
-  static constexpr int NXF = (TX + 1) * TY;  // x interfaces of the tile
-  static constexpr int NYF = TX * (TY + 1);  // y interfaces of the tile
-  static constexpr int RING = 4 * HX + 4 * TY;
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC>
+struct StripStage {
+  static constexpr int NW = STRIP_ROWS;
+  static constexpr int NT = 64 * NW;
+  static constexpr int HX = STRIP_W + 2;  // loaded columns: 64 lanes + 2 right halo columns
+  static constexpr int NQ = 7;            // primitive / temperature ring rows
+  static constexpr int NS = 5;            // y-slope and y-flux ring rows
+  static constexpr bool kTC = TC, kVISC = VISC;
 
   struct Smem {
-    R q[4][HY][HX];
-    R t[TC ? HY : 1][TC ? HX : 1];
-    R fx[4][TY][TX + 1];
-    R fy[4][TY + 1][TX];
-    R py[TY + 1][TX];
+    R q[4][NQ][HX];
+    R t[TC ? NQ : 1][TC ? HX : 1];
+    R sy[4][NS][STRIP_W];
+    R fy[4][NS][STRIP_W];
+    R pw[STRIP_W];  // interface pressure at the top of row Ny-2 (well-balanced wall flux)
   };
 
-  static FV2D_HD void put(Smem &sm, int lx, int ly, const St<R> &q) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) sm.q[v][ly][lx] = q.v[v];
-    if (TC) sm.t[TC ? ly : 0][TC ? lx : 0] = Ops<R>::div(q.v[IP], q.v[IR]);  // T = p / rho
-  }
+  // Ring slots. Rows are domain-relative (ghost rows negative / >= Ny); jb = segment start.
+  static FV2D_HD int qslot(int j, int jb) { return (j - jb + 2) % NQ; }
+  static FV2D_HD int sslot(int j, int jb) { return (j - jb + 1) % NS; }
 
   // Primitive state of domain-relative cell (gi, gj), ghosts included.
   static FV2D_HD void fetch_q(int gi, int gj, const KParams &kp, const StageIO<S> &io, St<R> &q) {
@@ -102,128 +147,209 @@ struct TileStage {
     if (fy) q.v[IV] = -q.v[IV];
   }
 
-  static FV2D_HD void ring_pos(int k, int &lx, int &ly) {
-    if (k < 2 * HX) {
-      ly = k / HX;
-      lx = k % HX;
-    } else if (k < 4 * HX) {
-      const int kk = k - 2 * HX;
-      ly = TY + 2 + kk / HX;
-      lx = kk % HX;
-    } else {
-      const int kk = k - 4 * HX;
-      ly = 2 + kk / 4;
-      const int c = kk % 4;
-      lx = (c < 2) ? c : TX + c;
-    }
+  static FV2D_HD void put(Smem &sm, int slot, int col, const St<R> &q) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sm.q[v][slot][col] = q.v[v];
+    if (TC) sm.t[TC ? slot : 0][TC ? col : 0] = Ops<R>::div(q.v[IP], q.v[IR]);  // T = p / rho
   }
 
-  // ---------------------------------------------------------------- phase 1: load
-  static FV2D_HD void phase_load(int tid, int bx, int by, Smem &sm, const KParams &kp, const StageIO<S> &io,
-                                 CellRegs &r) {
-    const int i0 = bx * TX, j0 = by * TY;
-    {
-      const int xl = tid % TX, yl = tid / TX;
-      const int gi = i0 + xl, gj = j0 + yl;
+  static FV2D_HD R Q(const Smem &sm, int v, int j, int col, int jb) { return sm.q[v][qslot(j, jb)][col]; }
+  static FV2D_HD R T(const Smem &sm, int j, int col, int jb) {
+    return sm.t[TC ? qslot(j, jb) : 0][TC ? col : 0];
+  }
+
+  // Load one strip row j into the ring: lane l -> strip column l; the two right halo
+  // columns 64, 65 of `nextra` rows starting at j go to lanes 0 .. 2*nextra-1 of wave `xw`.
+  static FV2D_HD void load_row(int lane, int w, int j, int i0, int jb, Smem &sm, const KParams &kp,
+                               const StageIO<S> &io) {
+    St<R> q;
+    fetch_q(i0 - 2 + lane, j, kp, io, q);
+    put(sm, qslot(j, jb), lane, q);
+    (void)w;
+  }
+  static FV2D_HD void load_extra(int lane, int j0, int nrows, int i0, int jb, Smem &sm, const KParams &kp,
+                                 const StageIO<S> &io) {
+    if (lane < 2 * nrows) {
+      const int j = j0 + (lane >> 1), col = STRIP_W + (lane & 1);
       St<R> q;
-      if (gi < kp.Nx && gj < kp.Ny) {
-        const long long k = (long long)gj * kp.pitch + gi;
-        St<R> u;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const S s = io.src[v * kp.plane + k];
-          r.u[v] = (double)s;
-          u.v[v] = R(s);
-        }
-        if (io.avg) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) r.u0[v] = (double)io.avg[v * kp.plane + k];
-        }
-        if (io.qsrc == QSRC_Q0) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) q.v[v] = R(io.q0[v * kp.plane + k]);
-        } else {
-          q = cons_to_prim(u, R(kp.gm1));
-          if (io.qsrc == QSRC_U_RESET) reset_negative(q, R(kp.eps_reset));
-        }
-      } else {
-        fetch_q(gi, gj, kp, io, q);
-      }
-      put(sm, xl + 2, yl + 2, q);
-    }
-    for (int k = tid; k < RING; k += NT) {
-      int lx, ly;
-      ring_pos(k, lx, ly);
-      St<R> q;
-      fetch_q(i0 + lx - 2, j0 + ly - 2, kp, io, q);
-      put(sm, lx, ly, q);
+      fetch_q(i0 - 2 + col, j, kp, io, q);
+      put(sm, qslot(j, jb), col, q);
     }
   }
 
-  // Face states of an interface from the four cells a | b || c | d along the sweep:
-  // left state = b's right face, right state = c's left face (reconstruct, Update.h:16-37,
-  // with the slopes of computeSlopes, Update.h:59-91).
-  static FV2D_HD void faces(R a, R b, R c, R d, R &fl, R &fr) {
-    if (RECON == PLM) {
-      const R sb = minmod(b - a, c - b);
-      const R sc = minmod(c - b, d - c);
-      fl = plm_face(b, sb, R(1));
-      fr = plm_face(c, sc, R(-1));
-    } else {
-      fl = b;
-      fr = c;
+  // Own-cell U (and U^n) of output lanes: issued at the start of a chunk, used by update.
+  static FV2D_HD void load_own(int lane, int j, int i0, const KParams &kp, const StageIO<S> &io, LaneRegs<R> &r) {
+    const int gi = i0 - 2 + lane;
+    if (lane >= 2 && gi < kp.Nx && j < kp.Ny) {
+      const long long k = (long long)j * kp.pitch + gi;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) r.u[v] = (double)io.src[v * kp.plane + k];
+      if (io.avg) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r.u0[v] = (double)io.avg[v * kp.plane + k];
+      }
     }
   }
 
-  // ---------------------------------------------------------------- phase 2: fluxes
-  static FV2D_HD void phase_flux(int tid, Smem &sm, const KParams &kp) {
-    const R gamma0 = R(kp.gamma0), gm1 = R(kp.gm1), entho = R(kp.entho), K = R(kp.fslp_K);
-    for (int k = tid; k < NXF + NYF; k += NT) {
-      if (k < NXF) {
-        // interface between local cells xi-1 and xi of row yl
-        const int xi = k % (TX + 1), yl = k / (TX + 1);
-        const int row = yl + 2, c0 = xi;  // halo columns c0 .. c0+3 hold a b c d
-        St<R> qL, qR;
+  static FV2D_HD R slope(R a, R b, R c) { return minmod(b - a, c - b); }
+
+  // y slope of (col, j) from rows j-1, j, j+1 of the ring.
+  static FV2D_HD void yslope(const Smem &sm, int j, int col, int jb, R s[4]) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          faces(sm.q[v][row][c0], sm.q[v][row][c0 + 1], sm.q[v][row][c0 + 2], sm.q[v][row][c0 + 3], qL.v[v],
-                qR.v[v]);
-        St<R> f;
-        R pout;
-        riemann<RIEM>(qL, qR, gamma0, gm1, entho, K, R(kp.gdx_x), f, pout);
+    for (int v = 0; v < 4; ++v) {
+      if (RECON == PLM)
+        s[v] = slope(Q(sm, v, j - 1, col, jb), Q(sm, v, j, col, jb), Q(sm, v, j + 1, col, jb));
+      else
+        s[v] = R(0);
+    }
+  }
+
+  // y interface between rows j and j+1 at column col, given both rows' y slopes; returns
+  // the flux in (rho, mx, my, E) order and the interface pressure.
+  static FV2D_HD void yflux(const Smem &sm, int j, int col, int jb, const R slo[4], const R shi[4],
+                            const KParams &kp, St<R> &f, R &pout) {
+    St<R> a, b;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sm.fx[v][yl][xi] = f.v[v];
-      } else {
-        // interface between local rows yi-1 and yi of column xl
-        const int kk = k - NXF;
-        const int xl = kk % TX, yi = kk / TX;
-        const int col = xl + 2, r0 = yi;
-        St<R> fL, fR;
+    for (int v = 0; v < 4; ++v) {
+      const R lo = Q(sm, v, j, col, jb), hi = Q(sm, v, j + 1, col, jb);
+      a.v[v] = (RECON == PLM) ? plm_face(lo, slo[v], R(1)) : lo;
+      b.v[v] = (RECON == PLM) ? plm_face(hi, shi[v], R(-1)) : hi;
+    }
+    St<R> fs;
+    riemann<RIEM>(swap_uv(a), swap_uv(b), R(kp.gamma0), R(kp.gm1), R(kp.entho), R(kp.fslp_K), R(kp.gdx_y), fs,
+                  pout);
+    f = swap_uv(fs);
+  }
+
+  // ---------------------------------------------------------------- segment prologue
+  // Rows jb-2 .. jb+1 into the ring (wave w: row jb-2+w; wave 0 also the right halo).
+  static FV2D_HD void pro_load(int lane, int w, int i0, int jb, Smem &sm, const KParams &kp, const StageIO<S> &io) {
+    load_row(lane, w, jb - 2 + w, i0, jb, sm, kp, io);
+    if (w == 0) load_extra(lane, jb - 2, 4, i0, jb, sm, kp, io);
+  }
+  // y slopes of rows jb-1 (wave 0) and jb (wave 1).
+  static FV2D_HD void pro_slopes(int lane, int w, int jb, Smem &sm) {
+    if (w < 2) {
+      R s[4];
+      yslope(sm, jb - 1 + w, lane, jb, s);
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          faces(sm.q[v][r0][col], sm.q[v][r0 + 1][col], sm.q[v][r0 + 2][col], sm.q[v][r0 + 3][col], fL.v[v],
-                fR.v[v]);
-        const St<R> qL = swap_uv(fL), qR = swap_uv(fR);
-        St<R> f;
-        R pout;
-        riemann<RIEM>(qL, qR, gamma0, gm1, entho, K, R(kp.gdx_y), f, pout);
-        const St<R> fb = swap_uv(f);
+      for (int v = 0; v < 4; ++v) sm.sy[v][sslot(jb - 1 + w, jb)][lane] = s[v];
+    }
+  }
+  // The bottom interface of the segment (rows jb-1, jb), by wave 0.
+  static FV2D_HD void pro_flux(int lane, int w, int jb, Smem &sm, const KParams &kp) {
+    if (w != 0) return;
+    R slo[4], shi[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sm.fy[v][yi][xl] = fb.v[v];
-        sm.py[yi][xl] = pout;
-      }
+    for (int v = 0; v < 4; ++v) {
+      slo[v] = sm.sy[v][sslot(jb - 1, jb)][lane];
+      shi[v] = sm.sy[v][sslot(jb, jb)][lane];
+    }
+    St<R> f;
+    R pout;
+    yflux(sm, jb - 1, lane, jb, slo, shi, kp, f, pout);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sm.fy[v][sslot(jb - 1, jb)][lane] = f.v[v];
+    if (jb - 1 == kp.Ny - 2) sm.pw[lane] = pout;
+  }
+
+  // ---------------------------------------------------------------- chunk phases
+  // Rows jc+2 .. jc+5 (wave w: jc+2+w); the right halo columns by wave `xw`.
+  static FV2D_HD void chunk_load(int lane, int w, int xw, int i0, int jb, int jc, Smem &sm, const KParams &kp,
+                                 const StageIO<S> &io, LaneRegs<R> &r) {
+    load_row(lane, w, jc + 2 + w, i0, jb, sm, kp, io);
+    if (w == xw) load_extra(lane, jc + 2, 4, i0, jb, sm, kp, io);
+    load_own(lane, jc + w, i0, kp, io, r);
+  }
+
+  // Pass 1a: x slope of column lane+1 of row jc+w, and the y slope of (lane, jc+w+1).
+  static FV2D_HD void pass1_slopes(int lane, int w, int jb, int jc, Smem &sm, LaneRegs<R> &r) {
+    const int j = jc + w;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      r.sx[v] = (RECON == PLM) ? slope(Q(sm, v, j, lane, jb), Q(sm, v, j, lane + 1, jb), Q(sm, v, j, lane + 2, jb))
+                               : R(0);
+    yslope(sm, j + 1, lane, jb, r.syu);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sm.sy[v][sslot(j + 1, jb)][lane] = r.syu[v];
+  }
+
+  // Pass 1b: the x interface (lane, lane+1) of row jc+w (needs sxl = slope of column lane).
+  static FV2D_HD void pass1_flux(int lane, int w, int jb, int jc, const Smem &sm, const KParams &kp,
+                                 LaneRegs<R> &r) {
+    const int j = jc + w;
+    St<R> a, b;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const R lo = Q(sm, v, j, lane, jb), hi = Q(sm, v, j, lane + 1, jb);
+      a.v[v] = (RECON == PLM) ? plm_face(lo, r.sxl[v], R(1)) : lo;
+      b.v[v] = (RECON == PLM) ? plm_face(hi, r.sx[v], R(-1)) : hi;
+    }
+    St<R> f;
+    R pout;
+    riemann<RIEM>(a, b, R(kp.gamma0), R(kp.gm1), R(kp.entho), R(kp.fslp_K), R(kp.gdx_x), f, pout);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r.fx[v] = f.v[v];
+  }
+
+  // Pass 2: the y interface (jc+w, jc+w+1) at column lane.
+  static FV2D_HD void pass2_flux(int lane, int w, int jb, int jc, Smem &sm, const KParams &kp, LaneRegs<R> &r) {
+    const int j = jc + w;
+    R slo[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) slo[v] = sm.sy[v][sslot(j, jb)][lane];
+    St<R> f;
+    yflux(sm, j, lane, jb, slo, r.syu, kp, f, r.py);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      r.fy[v] = f.v[v];
+      sm.fy[v][sslot(j, jb)][lane] = f.v[v];
+    }
+    if (j == kp.Ny - 2) sm.pw[lane] = r.py;
+  }
+
+  // Update part 1: conduction and viscous terms of the x face (lane, lane+1) of row jc+w.
+  static FV2D_HD void update_faces(int lane, int w, int jb, int jc, const Smem &sm, const KParams &kp,
+                                   LaneRegs<R> &r) {
+    const int j = jc + w;
+    if (TC) {
+      const R kc = R(kp.kappa);
+      const R kf = R(0.5) * (kc + kc);
+      r.tcx = Ops<R>::div_c(kf * (T(sm, j, lane + 1, jb) - T(sm, j, lane, jb)), R(kp.dx), R(kp.inv_dx), kp.fastdiv);
+    }
+    if (VISC) {
+      const R mu = R(kp.mu), odx = R(kp.inv_dx), ody = R(kp.inv_dy);
+      const int ca = lane, cb = lane + 1;
+      const R ub = Q(sm, IU, j, cb, jb), ua = Q(sm, IU, j, ca, jb);
+      const R vb = Q(sm, IV, j, cb, jb), va = Q(sm, IV, j, ca, jb);
+      const R qiu = (ub + ua) * R(0.5);
+      const R qiv = (vb + va) * R(0.5);
+      const R dudx = odx * (ub - ua);
+      const R dvdx = odx * (vb - va);
+      const R dudy = R(0.25) * ody *
+                     (Q(sm, IU, j + 1, cb, jb) - Q(sm, IU, j - 1, cb, jb) + Q(sm, IU, j + 1, ca, jb) -
+                      Q(sm, IU, j - 1, ca, jb));
+      const R dvdy = R(0.25) * ody *
+                     (Q(sm, IV, j + 1, cb, jb) - Q(sm, IV, j - 1, cb, jb) + Q(sm, IV, j + 1, ca, jb) -
+                      Q(sm, IV, j - 1, ca, jb));
+      const R tau_xx = R(4.0 / 3.0) * dudx - R(2.0 / 3.0) * dvdy;
+      const R tau_xy = dvdx + dudy;
+      r.vx[0] = mu * tau_xx;
+      r.vx[1] = mu * tau_xy;
+      r.vx[2] = mu * (tau_xx * qiu + tau_xy * qiv);
     }
   }
 
   // U += ((FL - FR) * dt) / d (Update.h:158-159; States.h:49-101 fix the order). In the
   // FP32 form the flux difference is applied in FP64 with a precomputed dt / d.
-  static FV2D_HD void add_flux(double acc[4], const St<R> &FL, const St<R> &FR, double dt, double dt_d, double d) {
+  static FV2D_HD void add_flux(double acc[4], const R FL[4], const R FR[4], double dt, double dt_d, double d,
+                               double rd, int fastdiv) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       if (sizeof(R) == 8)
-        acc[v] = acc[v] + ((double)(FL.v[v] - FR.v[v]) * dt) / d;
+        acc[v] = acc[v] + Ops<double>::div_c((double)(FL[v] - FR[v]) * dt, d, rd, fastdiv);
       else
-        acc[v] = acc[v] + ((double)FL.v[v] - (double)FR.v[v]) * dt_d;
+        acc[v] = acc[v] + ((double)FL[v] - (double)FR[v]) * dt_d;
     }
   }
 
@@ -234,16 +360,13 @@ struct TileStage {
     acc[IE] = acc[IE] + dt * 0.5 * ((double)FL0 + (double)FR0) * g;
   }
 
-  // ---------------------------------------------------------------- phase 3: update
-  static FV2D_HD void phase_update(int tid, int bx, int by, Smem &sm, const KParams &kp, const StageIO<S> &io,
-                                   double dt, double dt_dx, double dt_dy, CellRegs &r) {
-    const int xl = tid % TX, yl = tid / TX;
-    const int gi = bx * TX + xl, gj = by * TY + yl;
-    r.neg_rho = r.neg_prs = r.nan_cnt = 0;
-    r.inv = 0.0;
-    if (gi >= kp.Nx || gj >= kp.Ny) return;
-    const int cx = xl + 2, cy = yl + 2;
-    const R rho = sm.q[IR][cy][cx];
+  // Update part 2: the cell (lane, jc+w) (needs fxl, tcxl, vxl from lane-1).
+  static FV2D_HD void update_cell(int lane, int w, int i0, int jb, int jc, int je, const Smem &sm,
+                                  const KParams &kp, const StageIO<S> &io, double dt, double dt_dx, double dt_dy,
+                                  LaneRegs<R> &r) {
+    const int j = jc + w, gi = i0 - 2 + lane;
+    if (lane < 2 || gi >= kp.Nx || j >= je) return;
+    const R rho = Q(sm, IR, j, lane, jb);
 
     double acc[4];
 #pragma unroll
@@ -254,124 +377,97 @@ struct TileStage {
       gX = kp.gxf;
       gY = kp.gyf;
     } else if (kp.grav == GRAV_ANALYTICAL) {
-      gX = gY = (double)io.grav_row[gj];
+      gX = gY = (double)io.grav_row[j];
     }
 
     // x sweep (Update.h:171)
-    {
-      St<R> FL, FR;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        FL.v[v] = sm.fx[v][yl][xl];
-        FR.v[v] = sm.fx[v][yl][xl + 1];
-      }
-      add_flux(acc, FL, FR, dt, dt_dx, kp.dx);
-      if (kp.grav != GRAV_NONE) add_gravity(acc, rho, FL.v[IR], FR.v[IR], dt, gX);
-    }
+    add_flux(acc, r.fxl, r.fx, dt, dt_dx, kp.dx, kp.inv_dx, kp.fastdiv);
+    if (kp.grav != GRAV_NONE) add_gravity(acc, rho, r.fxl[IR], r.fx[IR], dt, gX);
+
     // y sweep (Update.h:172)
     {
-      St<R> FL, FR;
+      R FL[4], FR[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        FL.v[v] = sm.fy[v][yl][xl];
-        FR.v[v] = sm.fy[v][yl + 1][xl];
+        FL[v] = sm.fy[v][sslot(j - 1, jb)][lane];
+        FR[v] = r.fy[v];
       }
       // well-balanced wall flux (Update.h:149-156), with the pout of the other face
-      if (kp.wb && (gj == 0 || gj == kp.Ny - 1)) {
+      if (kp.wb && (j == 0 || j == kp.Ny - 1)) {
         const R g = R(gY);
-        if (gj == 0) {
-          FL.v[0] = R(0);
-          FL.v[1] = R(0);
-          FL.v[2] = sm.py[yl + 1][xl] - rho * g * R(kp.dy);
-          FL.v[3] = R(0);
+        if (j == 0) {
+          FL[0] = R(0);
+          FL[1] = R(0);
+          FL[2] = r.py - rho * g * R(kp.dy);
+          FL[3] = R(0);
         } else {
-          FR.v[0] = R(0);
-          FR.v[1] = R(0);
-          FR.v[2] = sm.py[yl][xl] + rho * g * R(kp.dy);
-          FR.v[3] = R(0);
+          FR[0] = R(0);
+          FR[1] = R(0);
+          FR[2] = sm.pw[lane] + rho * g * R(kp.dy);
+          FR[3] = R(0);
         }
       }
-      add_flux(acc, FL, FR, dt, dt_dy, kp.dy);
-      if (kp.grav != GRAV_NONE) add_gravity(acc, rho, FL.v[IR], FR.v[IR], dt, gY);
+      add_flux(acc, FL, FR, dt, dt_dy, kp.dy, kp.inv_dy, kp.fastdiv);
+      if (kp.grav != GRAV_NONE) add_gravity(acc, rho, FL[IR], FR[IR], dt, gY);
     }
 
     // thermal conduction into E (ThermalConduction.h:42-106)
     if (TC) {
       const R kc = R(kp.kappa);
       const R kf = R(0.5) * (kc + kc);
-      const R dx = R(kp.dx), dy = R(kp.dy);
-      const R Tc = sm.t[TC ? cy : 0][TC ? cx : 0];
-      const R TL = sm.t[TC ? cy : 0][TC ? cx - 1 : 0];
-      const R TR = sm.t[TC ? cy : 0][TC ? cx + 1 : 0];
-      const R TU = sm.t[TC ? cy - 1 : 0][TC ? cx : 0];
-      const R TD = sm.t[TC ? cy + 1 : 0][TC ? cx : 0];
-      R FL = Ops<R>::div(kf * (Tc - TL), dx);
-      R FR = Ops<R>::div(kf * (TR - Tc), dx);
-      const R FU = Ops<R>::div(kf * (Tc - TU), dy);
-      const R FD = Ops<R>::div(kf * (TD - Tc), dy);
+      const R dy = R(kp.dy);
+      const R Tc = T(sm, j, lane, jb);
+      R FL = r.tcxl;
+      R FR = r.tcx;
+      const R FU = Ops<R>::div_c(kf * (Tc - T(sm, j - 1, lane, jb)), dy, R(kp.inv_dy), kp.fastdiv);
+      const R FD = Ops<R>::div_c(kf * (T(sm, j + 1, lane, jb) - Tc), dy, R(kp.inv_dy), kp.fastdiv);
       // wall modes replace the x fluxes on the first / last row (ThermalConduction.h:75-103)
-      if (gj == 0 && kp.bctc_ymin != BCTC_NONE) {
+      if (j == 0 && kp.bctc_ymin != BCTC_NONE) {
         if (kp.bctc_ymin == BCTC_FIXED_TEMPERATURE)
-          FL = Ops<R>::div(kf * R(2) * (Tc - R(kp.bctc_ymin_value)), dy);
+          FL = Ops<R>::div_c(kf * R(2) * (Tc - R(kp.bctc_ymin_value)), dy, R(kp.inv_dy), kp.fastdiv);
         else if (kp.bctc_ymin == BCTC_FIXED_GRADIENT)
           FL = kf * R(kp.bctc_ymin_value);
       }
-      if (gj == kp.Ny - 1 && kp.bctc_ymax != BCTC_NONE) {
+      if (j == kp.Ny - 1 && kp.bctc_ymax != BCTC_NONE) {
         if (kp.bctc_ymax == BCTC_FIXED_TEMPERATURE)
-          FR = Ops<R>::div(kf * R(2) * (R(kp.bctc_ymax_value) - Tc), dy);
+          FR = Ops<R>::div_c(kf * R(2) * (R(kp.bctc_ymax_value) - Tc), dy, R(kp.inv_dy), kp.fastdiv);
         else if (kp.bctc_ymax == BCTC_FIXED_GRADIENT)
           FR = kf * R(kp.bctc_ymax_value);
       }
       acc[IE] = acc[IE] + (dt_dx * (double)(FR - FL) + dt_dy * (double)(FD - FU));
     }
 
-    // viscosity (Viscosity.h:33-117); stencil[r][c] = Q(i + c - 1, j + r - 1)
+    // viscosity (Viscosity.h:33-117): x faces shared with the neighbour lanes, y faces here
     if (VISC) {
-      const R mu = R(kp.mu);
-      const R odx = R(kp.inv_dx), ody = R(kp.inv_dy);
+      const R mu = R(kp.mu), odx = R(kp.inv_dx), ody = R(kp.inv_dy);
       const R four_thirds = R(4.0 / 3.0), two_thirds = R(2.0 / 3.0);
-#define FV2D_SQ(c, r, v) sm.q[v][cy + (r)-1][cx + (c)-1]
-      R fxu = R(0), fxv = R(0), fxe = R(0);
+      // x: flux = 0 + (-mu)*tau(left face) + (+mu)*tau(right face)
+      const R fxu = (R(0) + (-r.vxl[0])) + r.vx[0];
+      const R fxv = (R(0) + (-r.vxl[1])) + r.vx[1];
+      const R fxe = (R(0) + (-r.vxl[2])) + r.vx[2];
       R fyu = R(0), fyv = R(0), fye = R(0);
 #pragma unroll
       for (int side = 1; side < 3; ++side) {
         const R sign = (side == 1) ? R(-1) : R(1);
-        {  // x faces
-          const R qiu = (FV2D_SQ(side, 1, IU) + FV2D_SQ(side - 1, 1, IU)) * R(0.5);
-          const R qiv = (FV2D_SQ(side, 1, IV) + FV2D_SQ(side - 1, 1, IV)) * R(0.5);
-          const R dudx = odx * (FV2D_SQ(side, 1, IU) - FV2D_SQ(side - 1, 1, IU));
-          const R dvdx = odx * (FV2D_SQ(side, 1, IV) - FV2D_SQ(side - 1, 1, IV));
-          const R dudy = R(0.25) * ody *
-                         (FV2D_SQ(side, 2, IU) - FV2D_SQ(side, 0, IU) + FV2D_SQ(side - 1, 2, IU) -
-                          FV2D_SQ(side - 1, 0, IU));
-          const R dvdy = R(0.25) * ody *
-                         (FV2D_SQ(side, 2, IV) - FV2D_SQ(side, 0, IV) + FV2D_SQ(side - 1, 2, IV) -
-                          FV2D_SQ(side - 1, 0, IV));
-          const R tau_xx = four_thirds * dudx - two_thirds * dvdy;
-          const R tau_xy = dvdx + dudy;
-          fxu = fxu + sign * mu * tau_xx;
-          fxv = fxv + sign * mu * tau_xy;
-          fxe = fxe + sign * mu * (tau_xx * qiu + tau_xy * qiv);
-        }
-        {  // y faces
-          const R qiu = (FV2D_SQ(1, side, IU) + FV2D_SQ(1, side - 1, IU)) * R(0.5);
-          const R qiv = (FV2D_SQ(1, side, IV) + FV2D_SQ(1, side - 1, IV)) * R(0.5);
-          const R dudy = ody * (FV2D_SQ(1, side, IU) - FV2D_SQ(1, side - 1, IU));
-          const R dvdy = ody * (FV2D_SQ(1, side, IV) - FV2D_SQ(1, side - 1, IV));
-          const R dudx = R(0.25) * odx *
-                         (FV2D_SQ(2, side, IU) - FV2D_SQ(0, side, IU) + FV2D_SQ(2, side - 1, IU) -
-                          FV2D_SQ(0, side - 1, IU));
-          const R dvdx = R(0.25) * odx *
-                         (FV2D_SQ(2, side, IV) - FV2D_SQ(0, side, IV) + FV2D_SQ(2, side - 1, IV) -
-                          FV2D_SQ(0, side - 1, IV));
-          const R tau_yy = four_thirds * dvdy - two_thirds * dudx;
-          const R tau_xy = dvdx + dudy;
-          fyu = fyu + sign * mu * tau_xy;
-          fyv = fyv + sign * mu * tau_yy;
-          fye = fye + sign * mu * (tau_xy * qiu + tau_yy * qiv);
-        }
+        const int jh = j + side - 1, ja = j + side - 2;  // face between rows ja and jh
+        const R ub = Q(sm, IU, jh, lane, jb), ua = Q(sm, IU, ja, lane, jb);
+        const R vb = Q(sm, IV, jh, lane, jb), va = Q(sm, IV, ja, lane, jb);
+        const R qiu = (ub + ua) * R(0.5);
+        const R qiv = (vb + va) * R(0.5);
+        const R dudy = ody * (ub - ua);
+        const R dvdy = ody * (vb - va);
+        const R dudx = R(0.25) * odx *
+                       (Q(sm, IU, jh, lane + 1, jb) - Q(sm, IU, jh, lane - 1, jb) + Q(sm, IU, ja, lane + 1, jb) -
+                        Q(sm, IU, ja, lane - 1, jb));
+        const R dvdx = R(0.25) * odx *
+                       (Q(sm, IV, jh, lane + 1, jb) - Q(sm, IV, jh, lane - 1, jb) + Q(sm, IV, ja, lane + 1, jb) -
+                        Q(sm, IV, ja, lane - 1, jb));
+        const R tau_yy = four_thirds * dvdy - two_thirds * dudx;
+        const R tau_xy = dvdx + dudy;
+        fyu = fyu + sign * mu * tau_xy;
+        fyv = fyv + sign * mu * tau_yy;
+        fye = fye + sign * mu * (tau_xy * qiu + tau_yy * qiv);
       }
-#undef FV2D_SQ
       // U += dt * (vf_x + vf_y) (Viscosity.h:116); the density components are zero
       acc[IR] = acc[IR] + (double)(R(0) + R(0)) * dt;
       acc[IU] = acc[IU] + (double)(fxu + fyu) * dt;
@@ -385,7 +481,7 @@ struct TileStage {
       for (int v = 0; v < 4; ++v) acc[v] = 0.5 * (r.u0[v] + acc[v]);
     }
 
-    const long long k = (long long)gj * kp.pitch + gi;
+    const long long k = (long long)j * kp.pitch + gi;
     St<R> un;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -400,18 +496,16 @@ struct TileStage {
       const R eps = R(kp.eps_reset);
       if (q.v[IR] < R(0)) {
         q.v[IR] = eps;
-        r.neg_rho = 1;
+        r.neg_rho += 1;
       }
       if (q.v[IP] < R(0)) {
         q.v[IP] = eps;
-        r.neg_prs = 1;
+        r.neg_prs += 1;
       }
-      unsigned nn = 0;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) nn += (q.v[v] != q.v[v]) ? 1u : 0u;
-      r.nan_cnt = nn;
+      for (int v = 0; v < 4; ++v) r.nan_cnt += (q.v[v] != q.v[v]) ? 1u : 0u;
       const double inv = inv_dt_hyp(q, kp);
-      r.inv = (inv == inv) ? inv : 0.0;  // the reference's fmax-based Max ignores NaN
+      if (inv > r.inv) r.inv = inv;  // NaN never wins: the reference's fmax-based Max ignores it
     }
   }
 };

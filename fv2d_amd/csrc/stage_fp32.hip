@@ -5,9 +5,9 @@
 
 namespace fv2d {
 
-hipError_t launch_stage_fp32(const KParams &kp, const StageIO<float> &io, int recon, int riem, bool tc, bool visc,
-                             hipStream_t st) {
-  return launch_stage<float, float>(kp, io, recon, riem, tc, visc, st);
+hipError_t launch_stage_fp32(const KParams &kp, const StageIO<float> &io, const StripGeom &g, int recon, int riem,
+                             bool tc, bool visc, hipStream_t st) {
+  return launch_stage<float, float>(kp, io, g, recon, riem, tc, visc, st);
 }
 
 }  // namespace fv2d

@@ -92,7 +92,8 @@ typedef struct fv2d_params {
   double epsilon_reset_negative; /* value written over negative density / pressure */
   /* this build */
   int32_t precision; /* fv2d_precision */
-  int32_t reserved[7];
+  int32_t seg_rows;  /* rows per workgroup segment of the stage kernel, multiple of 4 (0: automatic) */
+  int32_t reserved[6];
 } fv2d_params;
 
 /* Per-step counters of the negative-value reset (SimInfo.h:604-606). */

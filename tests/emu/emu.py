@@ -33,6 +33,8 @@ def load(sanitized: bool = False):
     lib.emu_step.argtypes = [C.c_void_p, C.c_double, u64p]
     lib.emu_get_prim.argtypes = [C.c_void_p, dp]
     lib.emu_get_cons.argtypes = [C.c_void_p, dp]
+    lib.emu_markstein_mismatches.restype = C.c_longlong
+    lib.emu_markstein_mismatches.argtypes = [dp, C.c_longlong, C.c_double]
     return lib
 
 

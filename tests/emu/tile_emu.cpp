@@ -1,12 +1,13 @@
 // tile_emu.cpp -- TEST INFRASTRUCTURE: host emulation of the stage kernel.
 //
 // Runs the phase functions of fv2d_stage.h (the code the HIP kernel runs) on the CPU,
-// workgroup by workgroup and thread by thread, with the same step sequence as the C-ABI
-// layer (set_prim -> dt tiles -> reduce; step -> stages -> reduce). The LDS image of every
-// workgroup is filled with NaN before use, so a read of a slot that no thread wrote shows
-// up as a NaN in the results. Built with -fsanitize=address,undefined it also checks every
-// global and LDS index. Used by the CPU test suite to compare the kernel logic with the
-// oracle before any GPU run.
+// workgroup by workgroup and lane by lane, with the kernel's barriers as phase boundaries
+// and its DPP lane shifts as copies between lane records, and with the same step sequence
+// as the C-ABI layer (set_prim -> dt cells -> reduce; step -> stages -> reduce). The LDS
+// image of every workgroup is filled with NaN and the lane records with garbage before
+// use, so a read of anything not written first shows up in the results. Built with
+// -fsanitize=address,undefined it also checks every global and LDS index. Used by the CPU
+// test suite to compare the kernel logic with the oracle before any GPU run.
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -23,8 +24,6 @@ using namespace fv2d;
 
 namespace {
 
-constexpr int TX = 32, TY = 8;
-
 struct Sim {
   fv2d_params p;
   KParams kp;
@@ -33,48 +32,95 @@ struct Sim {
   std::vector<float> Uf, Usf, Q0f;
   std::vector<float> grav_row;
   std::vector<double> partials;
-  int gx, gy;
+  StripGeom g;
   StepCtl ctl;
   bool fresh = false;
   unsigned long long last[3] = {0, 0, 0};
 };
 
+// Lane l receives lane l-1's value; lane 0 gets 0 (the DPP wave_shr:1 of the kernel).
+template <typename R, typename F>
+void shift(std::vector<LaneRegs<R>> &regs, F field) {
+  for (int w = 0; w < STRIP_ROWS; ++w)
+    for (int l = 63; l >= 0; --l) {
+      LaneRegs<R> &dst = regs[w * 64 + l];
+      field(dst, l ? &regs[w * 64 + l - 1] : nullptr);
+    }
+}
+
 template <class K, typename S>
 void run_stage(Sim &s, const StageIO<S> &io) {
   const KParams &kp = s.kp;
-  std::vector<CellRegs> regs(K::NT);
+  typedef decltype(K::Smem::q[0][0][0]) RR;
+  typedef typename std::remove_const<typename std::remove_reference<RR>::type>::type R;
+  std::vector<LaneRegs<R>> regs(K::NT);
   typename K::Smem *sm = (typename K::Smem *)malloc(sizeof(typename K::Smem));
-  typedef decltype(sm->q[0][0][0]) RR;
-  typedef typename std::remove_reference<RR>::type R;
-  for (int by = 0; by < s.gy; ++by)
-    for (int bx = 0; bx < s.gx; ++bx) {
-      R *raw = (R *)sm;
-      for (size_t k = 0; k < sizeof(typename K::Smem) / sizeof(R); ++k) raw[k] = std::numeric_limits<R>::quiet_NaN();
-      for (int t = 0; t < K::NT; ++t) K::phase_load(t, bx, by, *sm, kp, io, regs[t]);
-      for (int t = 0; t < K::NT; ++t) K::phase_flux(t, *sm, kp);
-      for (int t = 0; t < K::NT; ++t)
-        K::phase_update(t, bx, by, *sm, kp, io, s.ctl.dt, s.ctl.dt_dx, s.ctl.dt_dy, regs[t]);
-      if (io.epilogue) {
-        double m = 0.0;
-        for (int t = 0; t < K::NT; ++t) {
-          m = fmax(m, regs[t].inv);
-          s.ctl.neg_rho += regs[t].neg_rho;
-          s.ctl.neg_prs += regs[t].neg_prs;
-          s.ctl.nan_cnt += regs[t].nan_cnt;
-        }
-        s.partials[bx + by * s.gx] = m;
-      }
+  const R nan = std::numeric_limits<R>::quiet_NaN();
+  for (int blk = 0; blk < s.g.nblocks; ++blk) {
+    int sx, sy;
+    strip_of_block(s.g, blk, sx, sy);
+    const int i0 = sx * STRIP_OUT, jb = sy * s.g.seg;
+    const int je = (jb + s.g.seg < kp.Ny) ? jb + s.g.seg : kp.Ny;
+    R *raw = (R *)sm;
+    for (size_t k = 0; k < sizeof(typename K::Smem) / sizeof(R); ++k) raw[k] = nan;
+    for (auto &r : regs) {
+      memset(&r, 0xff, sizeof r);  // garbage registers: every use must follow a definition
+      r.neg_rho = r.neg_prs = r.nan_cnt = 0;
+      r.inv = 0.0;
     }
+#define LANES(stmt)                                                                                                    \
+  for (int t = 0; t < K::NT; ++t) {                                                                                    \
+    const int lane = t & 63, w = t >> 6;                                                                               \
+    LaneRegs<R> &r = regs[t];                                                                                          \
+    (void)lane;                                                                                                        \
+    (void)w;                                                                                                           \
+    (void)r;                                                                                                           \
+    stmt;                                                                                                              \
+  }
+    LANES(K::pro_load(lane, w, i0, jb, *sm, kp, io));
+    LANES(K::pro_slopes(lane, w, jb, *sm));
+    LANES(K::pro_flux(lane, w, jb, *sm, kp));
+    int xw = 0;
+    for (int jc = jb; jc < je; jc += STRIP_ROWS) {
+      LANES(K::chunk_load(lane, w, xw, i0, jb, jc, *sm, kp, io, r));
+      LANES(K::pass1_slopes(lane, w, jb, jc, *sm, r));
+      shift(regs, [](LaneRegs<R> &d, const LaneRegs<R> *s_) {
+        for (int v = 0; v < 4; ++v) d.sxl[v] = s_ ? s_->sx[v] : R(0);
+      });
+      LANES(K::pass1_flux(lane, w, jb, jc, *sm, kp, r));
+      LANES(K::pass2_flux(lane, w, jb, jc, *sm, kp, r));
+      LANES(K::update_faces(lane, w, jb, jc, *sm, kp, r));
+      shift(regs, [](LaneRegs<R> &d, const LaneRegs<R> *s_) {
+        for (int v = 0; v < 4; ++v) d.fxl[v] = s_ ? s_->fx[v] : R(0);
+        if (K::kTC) d.tcxl = s_ ? s_->tcx : R(0);
+        if (K::kVISC)
+          for (int v = 0; v < 3; ++v) d.vxl[v] = s_ ? s_->vx[v] : R(0);
+      });
+      LANES(K::update_cell(lane, w, i0, jb, jc, je, *sm, kp, io, s.ctl.dt, s.ctl.dt_dx, s.ctl.dt_dy, r));
+      xw = (xw + 1) & (STRIP_ROWS - 1);
+    }
+#undef LANES
+    if (io.epilogue) {
+      double m = 0.0;
+      for (auto &r : regs) {
+        m = fmax(m, r.inv);
+        s.ctl.neg_rho += r.neg_rho;
+        s.ctl.neg_prs += r.neg_prs;
+        s.ctl.nan_cnt += r.nan_cnt;
+      }
+      s.partials[blk] = m;
+    }
+  }
   free(sm);
 }
 
 template <typename R, typename S, int RECON, int RIEM>
 void stage_rr(Sim &s, const StageIO<S> &io) {
   const bool tc = s.p.tc_active, vi = s.p.visc_active;
-  if (tc && vi) run_stage<TileStage<R, S, TX, TY, RECON, RIEM, true, true>, S>(s, io);
-  else if (tc) run_stage<TileStage<R, S, TX, TY, RECON, RIEM, true, false>, S>(s, io);
-  else if (vi) run_stage<TileStage<R, S, TX, TY, RECON, RIEM, false, true>, S>(s, io);
-  else run_stage<TileStage<R, S, TX, TY, RECON, RIEM, false, false>, S>(s, io);
+  if (tc && vi) run_stage<StripStage<R, S, RECON, RIEM, true, true>, S>(s, io);
+  else if (tc) run_stage<StripStage<R, S, RECON, RIEM, true, false>, S>(s, io);
+  else if (vi) run_stage<StripStage<R, S, RECON, RIEM, false, true>, S>(s, io);
+  else run_stage<StripStage<R, S, RECON, RIEM, false, false>, S>(s, io);
 }
 
 template <typename R, typename S, int RECON>
@@ -111,24 +157,25 @@ void reduce(Sim &s) {
 template <typename R, typename S>
 void dt_tiles(Sim &s, const S *src, int from_q0) {
   const KParams &kp = s.kp;
-  for (int by = 0; by < s.gy; ++by)
-    for (int bx = 0; bx < s.gx; ++bx) {
-      double m = 0.0;
-      for (int t = 0; t < TX * TY; ++t) {
-        const int gi = bx * TX + t % TX, gj = by * TY + t / TX;
-        if (gi >= kp.Nx || gj >= kp.Ny) continue;
-        const long long k = (long long)gj * kp.pitch + gi;
-        St<R> q;
-        for (int v = 0; v < 4; ++v) q.v[v] = R(src[v * kp.plane + k]);
-        if (!from_q0) {
-          q = cons_to_prim(q, R(kp.gm1));
-          reset_negative(q, R(kp.eps_reset));
-        }
-        const double inv = inv_dt_hyp(q, kp);
-        m = fmax(m, (inv == inv) ? inv : 0.0);
+  const long long n = (long long)kp.Nx * kp.Ny;
+  const int nparts = s.g.nblocks;
+  const long long per = (n + nparts - 1) / nparts;
+  for (int b = 0; b < nparts; ++b) {
+    double m = 0.0;
+    for (long long c = b * per; c < n && c < (b + 1) * per; ++c) {
+      const int j = (int)(c / kp.Nx), i = (int)(c % kp.Nx);
+      const long long k = (long long)j * kp.pitch + i;
+      St<R> q;
+      for (int v = 0; v < 4; ++v) q.v[v] = R(src[v * kp.plane + k]);
+      if (!from_q0) {
+        q = cons_to_prim(q, R(kp.gm1));
+        reset_negative(q, R(kp.eps_reset));
       }
-      s.partials[bx + by * s.gx] = m;
+      const double inv = inv_dt_hyp(q, kp);
+      if (inv > m) m = inv;
     }
+    s.partials[b] = m;
+  }
 }
 
 template <typename R, typename S>
@@ -204,6 +251,18 @@ void get_cons_t(const Sim &s, const std::vector<S> &U, double *u) {
 
 extern "C" {
 
+// Number of a[k] for which markstein_div(a, d, RN(1/d)) differs from a / d in any bit.
+long long emu_markstein_mismatches(const double *a, long long n, double d) {
+  const double rd = 1.0 / d;
+  const int fast = (d >= 0x1p-60 && d <= 0x1p+60) ? 1 : 0;
+  long long bad = 0;
+  for (long long k = 0; k < n; ++k) {
+    const double x = markstein_div(a[k], d, rd, fast), y = a[k] / d;
+    if (memcmp(&x, &y, sizeof x) != 0 && !(x != x && y != y)) ++bad;
+  }
+  return bad;
+}
+
 void *emu_create(const fv2d_params *p, char *err) {
   std::string why;
   if (check_params(p, why)) {
@@ -215,9 +274,8 @@ void *emu_create(const fv2d_params *p, char *err) {
   s->fp32 = p->precision == FV2D_FP32;
   const int pitch = (p->Nx + 63) / 64 * 64;
   build_kparams(p, pitch, s->kp);
-  s->gx = (p->Nx + TX - 1) / TX;
-  s->gy = (p->Ny + TY - 1) / TY;
-  s->partials.assign((size_t)s->gx * s->gy, 0.0);
+  s->g = strip_geom(p->Nx, p->Ny, pick_seg_rows(p));
+  s->partials.assign((size_t)s->g.nblocks, 0.0);
   const size_t n = 4 * (size_t)s->kp.plane;
   // unused padding columns hold NaN: a read of them would poison the results
   const double dn = std::numeric_limits<double>::quiet_NaN();
