@@ -53,7 +53,8 @@ __global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D
   const StepCtl *ctl = io.ctl;
   if (!ctl->active) return;  // uniform: the device loop has ended
   const double dt = ctl->dt, dt_dx = ctl->dt_dx, dt_dy = ctl->dt_dy;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: ring slots in SGPRs
   int sx, sy;
   strip_of_block(g, blockIdx.x, sx, sy);
   const int i0 = sx * STRIP_OUT, jb = sy * g.seg;
@@ -96,11 +97,20 @@ __global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D
   if (io.epilogue) commit_epilogue<K::NT>(r.neg_rho, r.neg_prs, r.nan_cnt, r.inv, io.ctl, io.partials, blockIdx.x);
 }
 
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC>
-hipError_t launch_stage_t(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
-  typedef StripStage<R, S, RECON, RIEM, TC, VISC> K;
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+hipError_t launch_stage_x(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
+  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG> K;
   hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
   return hipGetLastError();
+}
+
+// Gravity, the well-balanced wall flux and wall conduction modes need the XG variant.
+inline bool needs_xg(const KParams &kp) { return kp.grav != GRAV_NONE || kp.wb || kp.bctc_ymin || kp.bctc_ymax; }
+
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC>
+hipError_t launch_stage_t(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
+  if (needs_xg(kp)) return launch_stage_x<R, S, RECON, RIEM, TC, VISC, true>(kp, io, g, st);
+  return launch_stage_x<R, S, RECON, RIEM, TC, VISC, false>(kp, io, g, st);
 }
 
 template <typename R, typename S, int RECON, int RIEM>

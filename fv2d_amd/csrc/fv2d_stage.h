@@ -106,32 +106,39 @@ struct LaneRegs {
   double inv;
 };
 
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC>
+// XG: gravity, the well-balanced wall flux or wall conduction modes are on; without it
+// their code is compiled out.
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
 struct StripStage {
   static constexpr int NW = STRIP_ROWS;
   static constexpr int NT = 64 * NW;
   static constexpr int HX = STRIP_W + 2;  // loaded columns: 64 lanes + 2 right halo columns
   static constexpr int NQ = 7;            // primitive / temperature ring rows
-  static constexpr int NS = 5;            // y-slope and y-flux ring rows
+  static constexpr int NY = 6;            // y-slope / y-flux ring rows
+  static constexpr int NV = 5;            // viscous y-face ring rows
   static constexpr bool kTC = TC, kVISC = VISC;
 
   struct Smem {
     R q[4][NQ][HX];
     R t[TC ? NQ : 1][TC ? HX : 1];
-    R sy[4][NS][STRIP_W];
-    R fy[4][NS][STRIP_W];
+    // Slot of row j holds the y slope of row j until pass 2 of row j, then the flux at the
+    // top face of row j (written by the same wave, right after it reads the slope).
+    R yr[4][NY][STRIP_W];
+    // Viscous terms mu*tau_xy, mu*tau_yy, mu*(tau.u) at the top face of row j.
+    R vy[VISC ? 3 : 1][VISC ? NV : 1][STRIP_W];
     R pw[STRIP_W];  // interface pressure at the top of row Ny-2 (well-balanced wall flux)
   };
 
   // Ring slots. Rows are domain-relative (ghost rows negative / >= Ny); jb = segment start.
   static FV2D_HD int qslot(int j, int jb) { return (j - jb + 2) % NQ; }
-  static FV2D_HD int sslot(int j, int jb) { return (j - jb + 1) % NS; }
+  static FV2D_HD int yslot(int j, int jb) { return (j - jb + 1) % NY; }
+  static FV2D_HD int vslot(int j, int jb) { return (j - jb + 1) % NV; }
 
   // Primitive state of domain-relative cell (gi, gj), ghosts included.
   static FV2D_HD void fetch_q(int gi, int gj, const KParams &kp, const StageIO<S> &io, St<R> &q) {
-    bool fx, fy;
-    const int si = map_ghost(gi, kp.Nx, kp.bcx, fx);
-    const int sj = map_ghost(gj, kp.Ny, kp.bcy, fy);
+    bool fx = false, fy = false;
+    const int si = (gi >= 0 && gi < kp.Nx) ? gi : map_ghost(gi, kp.Nx, kp.bcx, fx);
+    const int sj = (gj >= 0 && gj < kp.Ny) ? gj : map_ghost(gj, kp.Ny, kp.bcy, fy);
     const long long k = (long long)sj * kp.pitch + si;
     if (io.qsrc == QSRC_Q0) {
 #pragma unroll
@@ -233,24 +240,48 @@ struct StripStage {
       R s[4];
       yslope(sm, jb - 1 + w, lane, jb, s);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sm.sy[v][sslot(jb - 1 + w, jb)][lane] = s[v];
+      for (int v = 0; v < 4; ++v) sm.yr[v][yslot(jb - 1 + w, jb)][lane] = s[v];
     }
   }
-  // The bottom interface of the segment (rows jb-1, jb), by wave 0.
+  // The bottom interface of the segment (rows jb-1, jb) and its viscous terms, by wave 0.
   static FV2D_HD void pro_flux(int lane, int w, int jb, Smem &sm, const KParams &kp) {
     if (w != 0) return;
     R slo[4], shi[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      slo[v] = sm.sy[v][sslot(jb - 1, jb)][lane];
-      shi[v] = sm.sy[v][sslot(jb, jb)][lane];
+      slo[v] = sm.yr[v][yslot(jb - 1, jb)][lane];
+      shi[v] = sm.yr[v][yslot(jb, jb)][lane];
     }
     St<R> f;
     R pout;
     yflux(sm, jb - 1, lane, jb, slo, shi, kp, f, pout);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) sm.fy[v][sslot(jb - 1, jb)][lane] = f.v[v];
-    if (jb - 1 == kp.Ny - 2) sm.pw[lane] = pout;
+    for (int v = 0; v < 4; ++v) sm.yr[v][yslot(jb - 1, jb)][lane] = f.v[v];
+    if (XG && jb - 1 == kp.Ny - 2) sm.pw[lane] = pout;
+    if (VISC) visc_yface(lane, jb - 1, jb, sm, kp);
+  }
+
+  // Viscous terms of the y face between rows j and j+1 at column lane (Viscosity.h:87-106,
+  // side 2 of row j = side 1 of row j+1), stored in the vy ring.
+  static FV2D_HD void visc_yface(int lane, int j, int jb, Smem &sm, const KParams &kp) {
+    const R mu = R(kp.mu), odx = R(kp.inv_dx), ody = R(kp.inv_dy);
+    const int cm = (lane > 0) ? lane - 1 : 0, cp = lane + 1;  // lane 0's face is never used
+    const R ub = Q(sm, IU, j + 1, lane, jb), ua = Q(sm, IU, j, lane, jb);
+    const R vb = Q(sm, IV, j + 1, lane, jb), va = Q(sm, IV, j, lane, jb);
+    const R qiu = (ub + ua) * R(0.5);
+    const R qiv = (vb + va) * R(0.5);
+    const R dudy = ody * (ub - ua);
+    const R dvdy = ody * (vb - va);
+    const R dudx = R(0.25) * odx *
+                   (Q(sm, IU, j + 1, cp, jb) - Q(sm, IU, j + 1, cm, jb) + Q(sm, IU, j, cp, jb) - Q(sm, IU, j, cm, jb));
+    const R dvdx = R(0.25) * odx *
+                   (Q(sm, IV, j + 1, cp, jb) - Q(sm, IV, j + 1, cm, jb) + Q(sm, IV, j, cp, jb) - Q(sm, IV, j, cm, jb));
+    const R tau_yy = R(4.0 / 3.0) * dvdy - R(2.0 / 3.0) * dudx;
+    const R tau_xy = dvdx + dudy;
+    const int sl = vslot(j, jb);
+    sm.vy[VISC ? 0 : 0][VISC ? sl : 0][lane] = mu * tau_xy;
+    sm.vy[VISC ? 1 : 0][VISC ? sl : 0][lane] = mu * tau_yy;
+    sm.vy[VISC ? 2 : 0][VISC ? sl : 0][lane] = mu * (tau_xy * qiu + tau_yy * qiv);
   }
 
   // ---------------------------------------------------------------- chunk phases
@@ -271,7 +302,7 @@ struct StripStage {
                                : R(0);
     yslope(sm, j + 1, lane, jb, r.syu);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) sm.sy[v][sslot(j + 1, jb)][lane] = r.syu[v];
+    for (int v = 0; v < 4; ++v) sm.yr[v][yslot(j + 1, jb)][lane] = r.syu[v];
   }
 
   // Pass 1b: the x interface (lane, lane+1) of row jc+w (needs sxl = slope of column lane).
@@ -292,20 +323,21 @@ struct StripStage {
     for (int v = 0; v < 4; ++v) r.fx[v] = f.v[v];
   }
 
-  // Pass 2: the y interface (jc+w, jc+w+1) at column lane.
+  // Pass 2: the y interface (jc+w, jc+w+1) at column lane, and its viscous terms.
   static FV2D_HD void pass2_flux(int lane, int w, int jb, int jc, Smem &sm, const KParams &kp, LaneRegs<R> &r) {
     const int j = jc + w;
     R slo[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) slo[v] = sm.sy[v][sslot(j, jb)][lane];
+    for (int v = 0; v < 4; ++v) slo[v] = sm.yr[v][yslot(j, jb)][lane];
     St<R> f;
     yflux(sm, j, lane, jb, slo, r.syu, kp, f, r.py);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       r.fy[v] = f.v[v];
-      sm.fy[v][sslot(j, jb)][lane] = f.v[v];
+      sm.yr[v][yslot(j, jb)][lane] = f.v[v];  // the slope of row j is no longer needed
     }
-    if (j == kp.Ny - 2) sm.pw[lane] = r.py;
+    if (XG && j == kp.Ny - 2) sm.pw[lane] = r.py;
+    if (VISC) visc_yface(lane, j, jb, sm, kp);
   }
 
   // Update part 1: conduction and viscous terms of the x face (lane, lane+1) of row jc+w.
@@ -373,27 +405,29 @@ struct StripStage {
     for (int v = 0; v < 4; ++v) acc[v] = r.u[v];
 
     double gX = 0.0, gY = 0.0;
-    if (kp.grav == GRAV_CONSTANT) {
-      gX = kp.gxf;
-      gY = kp.gyf;
-    } else if (kp.grav == GRAV_ANALYTICAL) {
-      gX = gY = (double)io.grav_row[j];
+    if (XG) {
+      if (kp.grav == GRAV_CONSTANT) {
+        gX = kp.gxf;
+        gY = kp.gyf;
+      } else if (kp.grav == GRAV_ANALYTICAL) {
+        gX = gY = (double)io.grav_row[j];
+      }
     }
 
     // x sweep (Update.h:171)
     add_flux(acc, r.fxl, r.fx, dt, dt_dx, kp.dx, kp.inv_dx, kp.fastdiv);
-    if (kp.grav != GRAV_NONE) add_gravity(acc, rho, r.fxl[IR], r.fx[IR], dt, gX);
+    if (XG && kp.grav != GRAV_NONE) add_gravity(acc, rho, r.fxl[IR], r.fx[IR], dt, gX);
 
     // y sweep (Update.h:172)
     {
       R FL[4], FR[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        FL[v] = sm.fy[v][sslot(j - 1, jb)][lane];
+        FL[v] = sm.yr[v][yslot(j - 1, jb)][lane];
         FR[v] = r.fy[v];
       }
       // well-balanced wall flux (Update.h:149-156), with the pout of the other face
-      if (kp.wb && (j == 0 || j == kp.Ny - 1)) {
+      if (XG && kp.wb && (j == 0 || j == kp.Ny - 1)) {
         const R g = R(gY);
         if (j == 0) {
           FL[0] = R(0);
@@ -408,7 +442,7 @@ struct StripStage {
         }
       }
       add_flux(acc, FL, FR, dt, dt_dy, kp.dy, kp.inv_dy, kp.fastdiv);
-      if (kp.grav != GRAV_NONE) add_gravity(acc, rho, FL[IR], FR[IR], dt, gY);
+      if (XG && kp.grav != GRAV_NONE) add_gravity(acc, rho, FL[IR], FR[IR], dt, gY);
     }
 
     // thermal conduction into E (ThermalConduction.h:42-106)
@@ -422,13 +456,13 @@ struct StripStage {
       const R FU = Ops<R>::div_c(kf * (Tc - T(sm, j - 1, lane, jb)), dy, R(kp.inv_dy), kp.fastdiv);
       const R FD = Ops<R>::div_c(kf * (T(sm, j + 1, lane, jb) - Tc), dy, R(kp.inv_dy), kp.fastdiv);
       // wall modes replace the x fluxes on the first / last row (ThermalConduction.h:75-103)
-      if (j == 0 && kp.bctc_ymin != BCTC_NONE) {
+      if (XG && j == 0 && kp.bctc_ymin != BCTC_NONE) {
         if (kp.bctc_ymin == BCTC_FIXED_TEMPERATURE)
           FL = Ops<R>::div_c(kf * R(2) * (Tc - R(kp.bctc_ymin_value)), dy, R(kp.inv_dy), kp.fastdiv);
         else if (kp.bctc_ymin == BCTC_FIXED_GRADIENT)
           FL = kf * R(kp.bctc_ymin_value);
       }
-      if (j == kp.Ny - 1 && kp.bctc_ymax != BCTC_NONE) {
+      if (XG && j == kp.Ny - 1 && kp.bctc_ymax != BCTC_NONE) {
         if (kp.bctc_ymax == BCTC_FIXED_TEMPERATURE)
           FR = Ops<R>::div_c(kf * R(2) * (R(kp.bctc_ymax_value) - Tc), dy, R(kp.inv_dy), kp.fastdiv);
         else if (kp.bctc_ymax == BCTC_FIXED_GRADIENT)
@@ -437,37 +471,17 @@ struct StripStage {
       acc[IE] = acc[IE] + (dt_dx * (double)(FR - FL) + dt_dy * (double)(FD - FU));
     }
 
-    // viscosity (Viscosity.h:33-117): x faces shared with the neighbour lanes, y faces here
+    // viscosity (Viscosity.h:33-117): every face term is computed once (x faces by the lane
+    // to the left of the face, y faces in pass 2); per cell and direction the flux is
+    // 0 + (-mu)*tau(low face) + (+mu)*tau(high face), as the reference accumulates it
     if (VISC) {
-      const R mu = R(kp.mu), odx = R(kp.inv_dx), ody = R(kp.inv_dy);
-      const R four_thirds = R(4.0 / 3.0), two_thirds = R(2.0 / 3.0);
-      // x: flux = 0 + (-mu)*tau(left face) + (+mu)*tau(right face)
+      const int lo = vslot(j - 1, jb), hi = vslot(j, jb);
       const R fxu = (R(0) + (-r.vxl[0])) + r.vx[0];
       const R fxv = (R(0) + (-r.vxl[1])) + r.vx[1];
       const R fxe = (R(0) + (-r.vxl[2])) + r.vx[2];
-      R fyu = R(0), fyv = R(0), fye = R(0);
-#pragma unroll
-      for (int side = 1; side < 3; ++side) {
-        const R sign = (side == 1) ? R(-1) : R(1);
-        const int jh = j + side - 1, ja = j + side - 2;  // face between rows ja and jh
-        const R ub = Q(sm, IU, jh, lane, jb), ua = Q(sm, IU, ja, lane, jb);
-        const R vb = Q(sm, IV, jh, lane, jb), va = Q(sm, IV, ja, lane, jb);
-        const R qiu = (ub + ua) * R(0.5);
-        const R qiv = (vb + va) * R(0.5);
-        const R dudy = ody * (ub - ua);
-        const R dvdy = ody * (vb - va);
-        const R dudx = R(0.25) * odx *
-                       (Q(sm, IU, jh, lane + 1, jb) - Q(sm, IU, jh, lane - 1, jb) + Q(sm, IU, ja, lane + 1, jb) -
-                        Q(sm, IU, ja, lane - 1, jb));
-        const R dvdx = R(0.25) * odx *
-                       (Q(sm, IV, jh, lane + 1, jb) - Q(sm, IV, jh, lane - 1, jb) + Q(sm, IV, ja, lane + 1, jb) -
-                        Q(sm, IV, ja, lane - 1, jb));
-        const R tau_yy = four_thirds * dvdy - two_thirds * dudx;
-        const R tau_xy = dvdx + dudy;
-        fyu = fyu + sign * mu * tau_xy;
-        fyv = fyv + sign * mu * tau_yy;
-        fye = fye + sign * mu * (tau_xy * qiu + tau_yy * qiv);
-      }
+      const R fyu = (R(0) + (-sm.vy[0][VISC ? lo : 0][lane])) + sm.vy[0][VISC ? hi : 0][lane];
+      const R fyv = (R(0) + (-sm.vy[VISC ? 1 : 0][VISC ? lo : 0][lane])) + sm.vy[VISC ? 1 : 0][VISC ? hi : 0][lane];
+      const R fye = (R(0) + (-sm.vy[VISC ? 2 : 0][VISC ? lo : 0][lane])) + sm.vy[VISC ? 2 : 0][VISC ? hi : 0][lane];
       // U += dt * (vf_x + vf_y) (Viscosity.h:116); the density components are zero
       acc[IR] = acc[IR] + (double)(R(0) + R(0)) * dt;
       acc[IU] = acc[IU] + (double)(fxu + fyu) * dt;

@@ -114,13 +114,20 @@ void run_stage(Sim &s, const StageIO<S> &io) {
   free(sm);
 }
 
+template <typename R, typename S, int RECON, int RIEM, bool XG>
+void stage_rrx(Sim &s, const StageIO<S> &io) {
+  const bool tc = s.p.tc_active, vi = s.p.visc_active;
+  if (tc && vi) run_stage<StripStage<R, S, RECON, RIEM, true, true, XG>, S>(s, io);
+  else if (tc) run_stage<StripStage<R, S, RECON, RIEM, true, false, XG>, S>(s, io);
+  else if (vi) run_stage<StripStage<R, S, RECON, RIEM, false, true, XG>, S>(s, io);
+  else run_stage<StripStage<R, S, RECON, RIEM, false, false, XG>, S>(s, io);
+}
+
 template <typename R, typename S, int RECON, int RIEM>
 void stage_rr(Sim &s, const StageIO<S> &io) {
-  const bool tc = s.p.tc_active, vi = s.p.visc_active;
-  if (tc && vi) run_stage<StripStage<R, S, RECON, RIEM, true, true>, S>(s, io);
-  else if (tc) run_stage<StripStage<R, S, RECON, RIEM, true, false>, S>(s, io);
-  else if (vi) run_stage<StripStage<R, S, RECON, RIEM, false, true>, S>(s, io);
-  else run_stage<StripStage<R, S, RECON, RIEM, false, false>, S>(s, io);
+  const KParams &kp = s.kp;
+  if (kp.grav != GRAV_NONE || kp.wb || kp.bctc_ymin || kp.bctc_ymax) stage_rrx<R, S, RECON, RIEM, true>(s, io);
+  else stage_rrx<R, S, RECON, RIEM, false>(s, io);
 }
 
 template <typename R, typename S, int RECON>
