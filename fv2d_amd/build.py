@@ -36,6 +36,7 @@ UNITS = [
     ("stage_fp64.hip", ["-ffp-contract=off"]),
     ("stage_fp32.hip", ["-ffp-contract=fast"]),
     ("fv2d_capi.hip", ["-ffp-contract=off"]),
+    ("fv2d_batch.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -28,6 +28,10 @@ FV2D_E_UNSUPPORTED = -2
 FV2D_E_HIP = -3
 FV2D_E_NOMEM = -4
 FV2D_E_STATE = -5
+FV2D_E_BUSY = -6
+
+PROB_SOD_X, PROB_BLAST, PROB_KH = 0, 1, 2
+JOB_FREE, JOB_RUNNING, JOB_DONE = 0, 1, 2
 
 
 class Params(C.Structure):
@@ -130,6 +134,65 @@ class StepStats(C.Structure):
         return (int(self.negative_density), int(self.negative_pressure), int(self.nan_count))
 
 
+class Problem(C.Structure):
+    """fv2d_problem: an initial condition generated on the device."""
+
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("seed", C.c_uint64),
+        ("amplitude", C.c_double),
+        ("reserved", C.c_double * 5),
+    ]
+
+
+class BatchConfig(C.Structure):
+    """fv2d_batch_config."""
+
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("max_jobs", C.c_int32),
+        ("arena_bytes", C.c_int64),
+        ("steps_per_launch", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("max_items", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+    @classmethod
+    def defaults(cls, **kw) -> "BatchConfig":
+        """The values of fv2d_batch_config_default, then the given overrides."""
+        c = cls()
+        c.precision = FP32
+        c.max_jobs = 256
+        c.arena_bytes = 8 << 30
+        c.steps_per_launch = 8
+        c.blocks_per_cu = 0
+        c.max_items = 1 << 18
+        c.use_graph = 1
+        for k, v in kw.items():
+            if not hasattr(c, k):
+                raise AttributeError(f"fv2d_batch_config has no field {k!r}")
+            setattr(c, k, v)
+        return c
+
+
+class JobInfo(C.Structure):
+    """fv2d_job_info."""
+
+    _fields_ = [
+        ("id", C.c_int64),
+        ("status", C.c_int32),
+        ("nsteps", C.c_int32),
+        ("t", C.c_double),
+        ("dt_last", C.c_double),
+        ("negative_density", C.c_uint64),
+        ("negative_pressure", C.c_uint64),
+        ("nan_count", C.c_uint64),
+    ]
+
+
 _LIB = None
 
 
@@ -172,6 +235,20 @@ def declare(lib: C.CDLL) -> None:
         "fv2d_get_stream": (C.c_void_p, [H]),
         "fv2d_profile": (C.c_int, [H, C.c_int]),
         "fv2d_profile_read": (C.c_int, [H, dp, P(C.c_int64)]),
+        "fv2d_set_problem": (C.c_int, [H, P(Problem)]),
+        "fv2d_batch_config_default": (None, [P(BatchConfig)]),
+        "fv2d_job_bytes": (C.c_int64, [P(Params)]),
+        "fv2d_batch_create": (C.c_int, [P(BatchConfig), P(H)]),
+        "fv2d_batch_destroy": (None, [H]),
+        "fv2d_batch_last_error": (C.c_char_p, [H]),
+        "fv2d_batch_submit": (C.c_int, [H, P(Params), P(Problem), dp, C.c_double, C.c_int32, P(C.c_int64)]),
+        "fv2d_batch_launch": (C.c_int, [H, C.c_int32]),
+        "fv2d_batch_wait": (C.c_int, [H, P(C.c_int32)]),
+        "fv2d_batch_poll": (C.c_int, [H, P(C.c_int64), C.c_int32, P(C.c_int32)]),
+        "fv2d_batch_job_info": (C.c_int, [H, C.c_int64, P(JobInfo)]),
+        "fv2d_batch_collect": (C.c_int, [H, C.c_int64, dp, dp, P(JobInfo)]),
+        "fv2d_batch_cancel": (C.c_int, [H, C.c_int64]),
+        "fv2d_batch_last_launch_ms": (C.c_int, [H, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -198,4 +275,18 @@ EXPORTED_SYMBOLS = (
     "fv2d_get_stream",
     "fv2d_profile",
     "fv2d_profile_read",
+    "fv2d_set_problem",
+    "fv2d_batch_config_default",
+    "fv2d_job_bytes",
+    "fv2d_batch_create",
+    "fv2d_batch_destroy",
+    "fv2d_batch_last_error",
+    "fv2d_batch_submit",
+    "fv2d_batch_launch",
+    "fv2d_batch_wait",
+    "fv2d_batch_poll",
+    "fv2d_batch_job_info",
+    "fv2d_batch_collect",
+    "fv2d_batch_cancel",
+    "fv2d_batch_last_launch_ms",
 )

@@ -41,22 +41,22 @@ __device__ inline void commit_epilogue(unsigned nr, unsigned np, unsigned nn, do
   }
 }
 
+// Ask the compiler for 4 waves per SIMD (at most 128 VGPRs): with a 40 KB LDS image, 4 workgroups
+// of 4 waves fill a CU; the FP64 KH variant would otherwise take 133 VGPRs and 3 waves.
 #ifndef FV2D_WAVES_PER_EU
 #define FV2D_WAVES_PER_EU 4
 #endif
 
-// Ask the compiler for 4 waves per SIMD (at most 128 VGPRs): with a 40 KB LDS image, 4 workgroups
-// of 4 waves fill a CU; the FP64 KH variant would otherwise take 133 VGPRs and 3 waves.
+// One workgroup's share of a stage: strip/segment `blk` of the simulation described by kp,
+// io, g (prologue, then the chunk loop; see fv2d_stage.h). Shared by the single-simulation
+// kernel and the batched (persistent, work-list) kernel.
 template <class K, typename R, typename S>
-__global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D_WAVES_PER_EU))) strip_kernel(const KParams kp, const StageIO<S> io, const StripGeom g) {
-  __shared__ typename K::Smem sm;
-  const StepCtl *ctl = io.ctl;
-  if (!ctl->active) return;  // uniform: the device loop has ended
-  const double dt = ctl->dt, dt_dx = ctl->dt_dx, dt_dy = ctl->dt_dy;
+__device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, const StripGeom &g, int blk, double dt,
+                                   double dt_dx, double dt_dy, typename K::Smem &sm) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: ring slots in SGPRs
   int sx, sy;
-  strip_of_block(g, blockIdx.x, sx, sy);
+  strip_of_block(g, blk, sx, sy);
   const int i0 = sx * STRIP_OUT, jb = sy * g.seg;
   const int je = (jb + g.seg < kp.Ny) ? jb + g.seg : kp.Ny;
 
@@ -94,7 +94,134 @@ __global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D
     __syncthreads();
     xw = (xw + 1) & (STRIP_ROWS - 1);
   }
-  if (io.epilogue) commit_epilogue<K::NT>(r.neg_rho, r.neg_prs, r.nan_cnt, r.inv, io.ctl, io.partials, blockIdx.x);
+  if (io.epilogue) commit_epilogue<K::NT>(r.neg_rho, r.neg_prs, r.nan_cnt, r.inv, io.ctl, io.partials, blk);
+}
+
+template <class K, typename R, typename S>
+__global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D_WAVES_PER_EU)))
+strip_kernel(const KParams kp, const StageIO<S> io, const StripGeom g) {
+  __shared__ typename K::Smem sm;
+  const StepCtl *ctl = io.ctl;
+  if (!ctl->active) return;  // uniform: the device loop has ended
+  strip_block<K, R, S>(kp, io, g, blockIdx.x, ctl->dt, ctl->dt_dx, ctl->dt_dy, sm);
+}
+
+// Batched stage: a persistent grid pulls (job, item) pairs of one kernel class from its run
+// [range[0], range[0] + range[1]) of the work list through an atomic counter and runs strip_block on them with each job's own
+// parameters, buffers and dt. stage = 1 or 2 (RK2); an Euler job does its whole step in
+// stage 1, with U / Us alternating by step parity. A job's arithmetic does not depend on
+// the other jobs, so it is bitwise the same alone or in the mix.
+template <class K, typename R, typename S>
+__global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D_WAVES_PER_EU)))
+batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restrict__ items, const int *range,
+                   int *counter, int stage) {
+  __shared__ typename K::Smem sm;
+  __shared__ int s_item;
+  const int base = range[0], n = range[1];  // this class's run of the work list
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);  // uniform: descriptor loads stay scalar
+    __syncthreads();
+    if (it >= n) break;
+    const BatchItem wi = items[base + it];
+    const BatchJob &J = jobs[wi.job];
+    StepCtl *ctl = J.ctl;
+    if (!ctl->active) continue;
+    if (!J.rk2 && stage == 2) continue;
+    StageIO<S> io;
+    io.q0 = (const S *)J.Q0;
+    io.grav_row = J.grav_row;
+    io.ctl = ctl;
+    io.partials = J.partials;
+    if (J.rk2) {
+      if (stage == 1) {
+        io.src = (const S *)J.U;
+        io.qsrc = ctl->fresh ? QSRC_Q0 : QSRC_U_RESET;
+        io.avg = nullptr;
+        io.out = (S *)J.Us;
+        io.epilogue = 0;
+      } else {
+        io.src = (const S *)J.Us;
+        io.qsrc = QSRC_U;
+        io.avg = (const S *)J.U;
+        io.out = (S *)J.U;
+        io.epilogue = 1;
+      }
+    } else {
+      const bool odd = ((ctl->nsteps - 1) & 1) != 0;  // nsteps counts the step being taken
+      io.src = (const S *)(odd ? J.Us : J.U);
+      io.out = (S *)(odd ? J.U : J.Us);
+      io.qsrc = ctl->fresh ? QSRC_Q0 : QSRC_U_RESET;
+      io.avg = nullptr;
+      io.epilogue = 1;
+    }
+    StripGeom g;
+    g.nsx = J.nsx;
+    g.nsy = J.nsy;
+    g.seg = J.seg;
+    g.nblocks = J.nblocks;
+    strip_block<K, R, S>(J.kp, io, g, wi.blk, ctl->dt, ctl->dt_dx, ctl->dt_dy, sm);
+    __syncthreads();
+  }
+}
+
+// Kernel class of a simulation: the template switches of the stage.
+struct StageClass {
+  int recon, riem, tc, visc, xg;
+  bool operator==(const StageClass &o) const {
+    return recon == o.recon && riem == o.riem && tc == o.tc && visc == o.visc && xg == o.xg;
+  }
+};
+
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+hipError_t launch_batch_x(const BatchJob *jobs, const BatchItem *items, const int *range, int *counter, int stage,
+                          int grid, hipStream_t st) {
+  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG> K;
+  hipLaunchKernelGGL((batch_stage_kernel<K, R, S>), dim3(grid), dim3(K::NT), 0, st, jobs, items, range, counter,
+                     stage);
+  return hipGetLastError();
+}
+
+// Resident workgroups per CU of a class's batched kernel (for the persistent grid size).
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+int batch_occupancy_x() {
+  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG> K;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_stage_kernel<K, R, S>, K::NT, 0) != hipSuccess) n = 1;
+  return n > 0 ? n : 1;
+}
+
+// Calls f.template operator()<RECON, RIEM, TC, VISC, XG>() for the class c.
+template <class F>
+auto with_class(const StageClass &c, F &&f) {
+#define FV2D_CLS(RC, RM)                                                                                               \
+  if (c.tc) {                                                                                                          \
+    if (c.visc) return c.xg ? f.template operator()<RC, RM, true, true, true>()                                        \
+                            : f.template operator()<RC, RM, true, true, false>();                                      \
+    return c.xg ? f.template operator()<RC, RM, true, false, true>() : f.template operator()<RC, RM, true, false, false>(); \
+  }                                                                                                                    \
+  if (c.visc) return c.xg ? f.template operator()<RC, RM, false, true, true>()                                         \
+                          : f.template operator()<RC, RM, false, true, false>();                                       \
+  return c.xg ? f.template operator()<RC, RM, false, false, true>() : f.template operator()<RC, RM, false, false, false>();
+  if (c.recon == PLM) {
+    if (c.riem == HLL) {
+      FV2D_CLS(PLM, HLL)
+    } else if (c.riem == FSLP) {
+      FV2D_CLS(PLM, FSLP)
+    } else {
+      FV2D_CLS(PLM, HLLC)
+    }
+  } else {
+    if (c.riem == HLL) {
+      FV2D_CLS(PCM, HLL)
+    } else if (c.riem == FSLP) {
+      FV2D_CLS(PCM, FSLP)
+    } else {
+      FV2D_CLS(PCM, HLLC)
+    }
+  }
+#undef FV2D_CLS
 }
 
 template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
@@ -146,6 +273,12 @@ hipError_t launch_stage(const KParams &kp, const StageIO<S> &io, const StripGeom
 }
 
 // Defined in stage_fp64.hip / stage_fp32.hip.
+hipError_t launch_batch_fp64(const StageClass &c, const BatchJob *jobs, const BatchItem *items, const int *range,
+                             int *counter, int stage, int grid, hipStream_t st);
+hipError_t launch_batch_fp32(const StageClass &c, const BatchJob *jobs, const BatchItem *items, const int *range,
+                             int *counter, int stage, int grid, hipStream_t st);
+int batch_occupancy_fp64(const StageClass &c);
+int batch_occupancy_fp32(const StageClass &c);
 hipError_t launch_stage_fp64(const KParams &kp, const StageIO<double> &io, const StripGeom &g, int recon, int riem,
                              bool tc, bool visc, hipStream_t st);
 hipError_t launch_stage_fp32(const KParams &kp, const StageIO<float> &io, const StripGeom &g, int recon, int riem,

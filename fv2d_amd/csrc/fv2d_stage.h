@@ -39,7 +39,7 @@ struct StepCtl {
   int active;               // stages run only while set
   int nsteps;               // steps taken in the current advance call
   int max_steps;            // advance limit (<= 0: none)
-  int pad;
+  int fresh;                // batch mode: the next first stage reads Q0
   unsigned long long neg_rho, neg_prs, nan_cnt;  // running totals of the reset
 };
 
@@ -56,6 +56,27 @@ struct StageIO {
   double *partials;       // per-workgroup max inverse dt (epilogue)
   int qsrc;               // QSRC_*
   int epilogue;
+};
+
+// One job of a batch (device-resident descriptor). All jobs of a batch share the
+// precision; buffers are typed by it.
+struct BatchJob {
+  KParams kp;
+  int nblocks;   // workgroup items of this job (strip geometry)
+  int seg;       // rows per segment
+  int nsx, nsy;
+  void *U, *Us, *Q0;
+  const float *grav_row;
+  StepCtl *ctl;
+  double *partials;
+  int rk2;       // SSP-RK2 (else Euler)
+  int used;      // slot holds a job
+  int cls;       // kernel class (index into the batch's class table)
+};
+
+struct BatchItem {
+  int job;  // slot index
+  int blk;  // workgroup item of the job
 };
 
 // Strip / segment geometry shared by the launcher, the kernel and the emulation.

@@ -38,6 +38,7 @@ extern "C" {
 #define FV2D_E_HIP (-3)         /* HIP runtime failure */
 #define FV2D_E_NOMEM (-4)       /* device or host allocation failed */
 #define FV2D_E_STATE (-5)       /* call made in the wrong order (e.g. step before set_prim) */
+#define FV2D_E_BUSY (-6)        /* batch: no free slot or arena space for the job right now */
 
 /* Option values: same members and order as the reference enums (SimInfo.h:41-97). */
 enum fv2d_riemann { FV2D_HLL = 0, FV2D_HLLC = 1, FV2D_FSLP = 2 };
@@ -176,6 +177,111 @@ void *fv2d_get_stream(fv2d_handle *h);
  * and the number of timed stage launches since the last read, then resets them. */
 int fv2d_profile(fv2d_handle *h, int enable);
 int fv2d_profile_read(fv2d_handle *h, double *stage_ms, int64_t *stage_launches);
+
+/* ------------------------------------------------------------------------------------
+ * Initial conditions generated on the device.
+ *
+ * The problems of the benchmark and serving workloads, evaluated per cell on the GPU so a
+ * job needs no host upload: FV2D_PROB_SOD_X (initSodX, Init.h:21-41: rho 1 / 0.125, p 1 /
+ * 0.1, split at x = 0.5), FV2D_PROB_BLAST (the Sedov-type point blast: rho 1, p 1e-5, and
+ * p = (gamma-1) E / (pi r0^2) with E = 1, r0 = 0.02 in cells whose centre lies within r0 of
+ * the domain centre; initBlast, Init.h:104-135, puts a fixed pressure in a disc instead) and
+ * FV2D_PROB_KH (Kelvin-Helmholtz: band 0.25 < y < 0.75 with rho 2, u 0.5, else rho 1,
+ * u -0.5, p 2.5, v = amplitude sin(4 pi x) + 1e-3 (2 U - 1), U a counter-based uniform
+ * keyed by seed and cell; initKelvinHelmholtz, Init.h:246-287, draws its noise from a
+ * Kokkos random pool instead). Cell centres as getPos (SimInfo.h:495-499).
+ * ------------------------------------------------------------------------------------ */
+enum fv2d_problem_kind { FV2D_PROB_SOD_X = 0, FV2D_PROB_BLAST = 1, FV2D_PROB_KH = 2 };
+
+typedef struct fv2d_problem {
+  int32_t kind; /* fv2d_problem_kind */
+  int32_t reserved0;
+  uint64_t seed;    /* FV2D_PROB_KH noise */
+  double amplitude; /* FV2D_PROB_KH: amplitude of the sin(4 pi x) perturbation of v */
+  double reserved[5];
+} fv2d_problem;
+
+/* Replaces: InitFunctor::init (Init.h:310-358) followed by primToCons (main.cpp:58) for the
+ * problems above, on the device. Same contract as fv2d_set_prim afterwards. */
+int fv2d_set_problem(fv2d_handle *h, const fv2d_problem *prob);
+
+/* ------------------------------------------------------------------------------------
+ * Batched serving (no reference counterpart: the reference runs one simulation per
+ * process). A batch holds many independent simulations ("jobs") of one precision on one
+ * GPU. Each job keeps its state, its dt and its step counter on the device until its end
+ * (t_end or a step count); a launch advances every running job by up to a fixed number of
+ * steps, whatever step each job is at. The stage kernels are persistent: their workgroups
+ * pull (job, strip segment) items of one kernel class from a device work list, so jobs of
+ * any size share one launch. Jobs are admitted while a slot and room in the batch's HBM
+ * arena are free (FV2D_E_BUSY otherwise). A job's results do not depend on the other jobs
+ * in the batch: they equal the same job run alone, bit for bit.
+ *
+ * Threading: one host thread drives a batch. fv2d_batch_launch queues work and returns;
+ * fv2d_batch_wait blocks until it is done. Submit and collect may be called while a launch
+ * is in flight: their device work is ordered after it, and the collected job has finished.
+ * ------------------------------------------------------------------------------------ */
+typedef struct fv2d_batch fv2d_batch;
+
+typedef struct fv2d_batch_config {
+  int32_t precision;        /* fv2d_precision of every job in the batch */
+  int32_t max_jobs;         /* job slots */
+  int64_t arena_bytes;      /* device memory for job state; admission bound */
+  int32_t steps_per_launch; /* steps per launch (captured once as a HIP graph) */
+  int32_t blocks_per_cu;    /* persistent workgroups per CU (0: the occupancy limit) */
+  int32_t max_items;        /* work-list capacity in (job, segment) items */
+  int32_t use_graph;        /* 1: replay a captured graph; 0: launch the kernels directly */
+  int32_t reserved[8];
+} fv2d_batch_config;
+
+enum fv2d_job_status { FV2D_JOB_FREE = 0, FV2D_JOB_RUNNING = 1, FV2D_JOB_DONE = 2 };
+
+typedef struct fv2d_job_info {
+  int64_t id;
+  int32_t status; /* fv2d_job_status */
+  int32_t nsteps; /* steps taken */
+  double t;       /* simulation time reached */
+  double dt_last; /* last dt taken */
+  uint64_t negative_density, negative_pressure, nan_count;
+} fv2d_job_info;
+
+void fv2d_batch_config_default(fv2d_batch_config *c);
+
+/* Device bytes one job of these parameters takes from a batch arena. */
+int64_t fv2d_job_bytes(const fv2d_params *p);
+
+int fv2d_batch_create(const fv2d_batch_config *c, fv2d_batch **b);
+void fv2d_batch_destroy(fv2d_batch *b);
+const char *fv2d_batch_last_error(const fv2d_batch *b);
+
+/* Admit a job: parameters p (precision must be the batch's), initial state from prob
+ * (device-generated) or from q (4*Nx*Ny primitives, snapshot order; exactly one of the two
+ * non-NULL), run until t + epsilon >= tend or max_steps steps (<= 0: no limit), the loop
+ * rule of main.cpp:62-84. FV2D_E_BUSY if no slot or arena space is free now. */
+int fv2d_batch_submit(fv2d_batch *b, const fv2d_params *p, const fv2d_problem *prob, const double *q,
+                      double tend, int32_t max_steps, int64_t *job_id);
+
+/* Queue `launches` launches (each up to steps_per_launch steps of every running job). */
+int fv2d_batch_launch(fv2d_batch *b, int32_t launches);
+
+/* Wait for queued launches; *running = jobs still running (may be NULL). */
+int fv2d_batch_wait(fv2d_batch *b, int32_t *running);
+
+/* Ids of finished, not yet collected jobs (as of the last wait): up to cap into ids, count
+ * into *n. */
+int fv2d_batch_poll(fv2d_batch *b, int64_t *ids, int32_t cap, int32_t *n);
+
+/* Status of a job (as of the last wait). */
+int fv2d_batch_job_info(fv2d_batch *b, int64_t id, fv2d_job_info *info);
+
+/* Results of a finished job: primitives q (4*Nx*Ny, snapshot order; may be NULL), the sums
+ * of fv2d_sums (may be NULL) and its info (may be NULL). Frees the job's slot and memory. */
+int fv2d_batch_collect(fv2d_batch *b, int64_t id, double *q, double sums[4], fv2d_job_info *info);
+
+/* Drop a job (running or finished) without results. */
+int fv2d_batch_cancel(fv2d_batch *b, int64_t id);
+
+/* Duration (ms) of the last waited-for launches, from HIP events on the batch stream. */
+int fv2d_batch_last_launch_ms(fv2d_batch *b, double *ms);
 
 #ifdef __cplusplus
 }

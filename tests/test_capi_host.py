@@ -72,3 +72,57 @@ def test_create_without_gpu_fails_cleanly():
     rc = lib.fv2d_create(C.byref(Params.defaults()), C.byref(h))
     assert rc in (capi.FV2D_E_HIP, capi.FV2D_E_NOMEM) and not h.value
     assert lib.fv2d_last_error(None)
+
+
+def test_batch_config_default_and_job_bytes():
+    lib = capi.load_library()
+    c = capi.BatchConfig()
+    lib.fv2d_batch_config_default(C.byref(c))
+    d = capi.BatchConfig.defaults()
+    for name, _ in capi.BatchConfig._fields_:
+        if name != "reserved":
+            assert getattr(c, name) == getattr(d, name), name
+    # U, Us, Q0 (4 planes of pitch-64 rows) + one dt partial per work item, 256-byte aligned
+    p = Params.defaults(Nx=100, Ny=30, precision=capi.FP32)
+    state = -(-(4 * 128 * 30 * 4) // 256) * 256
+    assert lib.fv2d_job_bytes(C.byref(p)) == 3 * state + 256  # 2 strips x 1 segment: 16 B of partials
+    p64 = Params.defaults(Nx=4096, Ny=4096)
+    nblk = (-(-4096 // 62)) * (4096 // 64)
+    assert lib.fv2d_job_bytes(C.byref(p64)) == 3 * 4 * 4096 * 4096 * 8 + -(-(8 * nblk) // 256) * 256
+    # invalid parameters have no footprint
+    assert lib.fv2d_job_bytes(C.byref(Params.defaults(Nx=1))) == -1
+
+
+@pytest.mark.parametrize("kw,frag", [
+    (dict(precision=5), "precision"),
+    (dict(max_jobs=0), "max_jobs"),
+    (dict(max_jobs=2000), "max_jobs"),
+    (dict(steps_per_launch=0), "steps_per_launch"),
+    (dict(arena_bytes=16), "arena_bytes"),
+])
+def test_batch_config_rejected(kw, frag):
+    lib = capi.load_library()
+    c = capi.BatchConfig.defaults(**kw)
+    b = C.c_void_p()
+    assert lib.fv2d_batch_create(C.byref(c), C.byref(b)) == capi.FV2D_E_INVALID and not b.value
+    assert frag in lib.fv2d_batch_last_error(None).decode()
+
+
+def test_batch_null_handles():
+    lib = capi.load_library()
+    assert lib.fv2d_batch_launch(None, 1) == capi.FV2D_E_INVALID
+    assert lib.fv2d_batch_wait(None, None) == capi.FV2D_E_INVALID
+    assert lib.fv2d_batch_collect(None, 1, None, None, None) == capi.FV2D_E_INVALID
+    lib.fv2d_batch_destroy(None)
+    assert lib.fv2d_set_problem(None, None) == capi.FV2D_E_INVALID
+
+
+def test_batch_create_without_gpu_fails_cleanly():
+    from fv2d_amd import hiprt
+    if hiprt.device_count() > 0:
+        pytest.skip("a GPU is present")
+    lib = capi.load_library()
+    b = C.c_void_p()
+    rc = lib.fv2d_batch_create(C.byref(capi.BatchConfig.defaults()), C.byref(b))
+    assert rc in (capi.FV2D_E_HIP, capi.FV2D_E_NOMEM) and not b.value
+    assert lib.fv2d_batch_last_error(None)
