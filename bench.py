@@ -2,6 +2,7 @@
 """Benchmark of the explicit 2D hydro step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--cpu-seconds S]
+                    [--serve-jobs J] [--serve-steps S] [--serve-load L] [--no-serve]
 
 Workload: the Kelvin-Helmholtz shear layer at N x N (default 4096^2), PLM-minmod / HLLC /
 SSP-RK2, CFL 0.8, periodic, viscosity mu = 1e-4 and thermal conduction kappa = 1e-4 on,
@@ -18,6 +19,14 @@ achieved = bytes per launch x cells / mean launch time (HIP events around every 
 launch in the timed region). traffic = measured HBM bytes per launch (rocprofv3
 FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) from the committed summary
 under profiles/, when present.
+
+serving: the served job stream (2000 jobs of seven grid sizes from 128^2 to 2048x512,
+Sod / point blast / Kelvin-Helmholtz, seeded gamma and amplitude, 200 RK2 steps each, FP32
+form) through the batch engine with continuous batching, against the same jobs one at a
+time through the single-simulation path: aggregate Gcell-updates/s of each (all jobs
+queued at t = 0), p50/p99 latency (arrival to result) under seeded Poisson arrivals at
+--serve-load x the batched capacity, and a 5% seeded subsample re-run in FP64 with the
+FP32 form's error against it.
 
 cpu_baseline: the CPU oracle (a plain-C FP64 restatement of the reference path with the
 reference's pass structure, OpenMP) timed on the host cores on a bounded sample of the
@@ -119,6 +128,76 @@ def run_cpu(size: int, seconds: float, q0: np.ndarray):
     return dict(value=size * size * n / el / 1e9, steps=n, seconds=el, threads=lib.orc_get_threads())
 
 
+def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: int = 2024):
+    """The served job stream: autotune, batched and one-at-a-time capacity (all jobs queued
+    at t = 0), batched latency under Poisson arrivals at `load` x the batched capacity, and
+    the FP64 re-run of a seeded subsample as the accuracy check of the FP32 form."""
+    from fv2d_amd import serve as S
+    from fv2d_amd.capi import FP32, FP64, BatchConfig
+
+    t0 = time.perf_counter()
+    tune = S.autotune(FP32, segs=(32, 64), blocks_per_cu=(0,), log=log)
+    t_tune = time.perf_counter() - t0
+    specs = S.make_stream(n_jobs, rate=0, seed=seed, steps=steps)
+    rng = np.random.default_rng(seed + 1)
+    check = set(int(i) for i in rng.choice(n_jobs, max(1, int(round(check_frac * n_jobs))), replace=False))
+    cfg = BatchConfig.defaults(precision=FP32, max_jobs=128)
+    log(f"bench: serving {n_jobs} jobs x {steps} RK2 steps, batched (all queued at t=0)")
+    rb, prim32 = S.serve_batched(specs, FP32, cfg, tuning=tune, keep_prim=check, realtime=False)
+    log(f"bench: batched {rb.throughput / 1e9:.2f} Gcell/s, makespan {rb.makespan:.2f} s")
+    log("bench: serving one at a time (single-simulation path)")
+    rs, _ = S.serve_sequential(specs, FP32)
+    log(f"bench: one-at-a-time {rs.throughput / 1e9:.2f} Gcell/s, makespan {rs.makespan:.2f} s")
+    cap_jobs = n_jobs / rb.makespan
+    rate = load * cap_jobs
+    specs_rt = S.with_arrivals(specs, rate, seed=seed + 2)
+    log(f"bench: batched under Poisson arrivals at {rate:.1f} jobs/s ({load:.2f} x capacity)")
+    rr, _ = S.serve_batched(specs_rt, FP32, cfg, tuning=tune, realtime=True)
+    lat_b = rr.latency_percentiles()
+    service = {r.idx: r.finished - r.admitted for r in rs.records}
+    lat_s = S.fifo_latencies(specs_rt, service)
+    log(f"bench: FP64 re-run of {len(check)} jobs")
+    sub = [s for s in specs if s.idx in check]
+    r64, prim64 = S.serve_batched(sub, FP64, BatchConfig.defaults(precision=FP64, max_jobs=64), keep_prim=check,
+                                  realtime=False)
+    steps32 = {r.idx: r.nsteps for r in rb.records}
+    steps64 = {r.idx: r.nsteps for r in r64.records}
+    errs = {name: [] for name in ("rho", "u", "v", "prs")}
+    for i in sorted(check):
+        e = S.field_errors(prim32[i], prim64[i])
+        for name in errs:
+            errs[name].append(e[name])
+    fp64_check = {
+        "jobs": len(check),
+        "same_step_count": all(steps32[i] == steps64[i] for i in check),
+        "rel_L1_max": {k: float(max(v[0] for v in errs[k])) for k in errs},
+        "rel_L1_mean": {k: float(np.mean([v[0] for v in errs[k]])) for k in errs},
+        "rel_Linf_max": {k: float(max(v[1] for v in errs[k])) for k in errs},
+    }
+    return {
+        "jobs": n_jobs,
+        "steps_per_job": steps,
+        "cell_updates": rb.cell_steps,
+        "precision": "fp32 form",
+        "value": round(rb.throughput / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "one_at_a_time": round(rs.throughput / 1e9, 4),
+        "speedup_vs_one_at_a_time": round(rb.throughput / rs.throughput, 3),
+        "makespan_s": {"batched": round(rb.makespan, 3), "one_at_a_time": round(rs.makespan, 3)},
+        "gpu_busy_frac": round(rb.gpu_ms / 1e3 / rb.makespan, 3),
+        "arrival_rate_jobs_per_s": round(rate, 2),
+        "load": load,
+        "latency_ms": {"p50": round(lat_b["p50"] * 1e3, 2), "p99": round(lat_b["p99"] * 1e3, 2)},
+        "one_at_a_time_latency_ms": {"p50": round(float(np.percentile(lat_s, 50)) * 1e3, 2),
+                                     "p99": round(float(np.percentile(lat_s, 99)) * 1e3, 2),
+                                     "note": "single FIFO server with the measured one-at-a-time service times "
+                                             "under the same arrivals"},
+        "latency_all_queued_ms": {k: round(v * 1e3, 2) for k, v in rb.latency_percentiles().items()},
+        "fp64_check": fp64_check,
+        "autotune": {"seconds": round(t_tune, 1), "seg": tune["seg"]},
+    }
+
+
 def calibrate_copy(mib: int, reps: int = 4) -> int:
     """hipMemcpy device-to-device of `mib` MiB, `reps` times: a copy kernel that reads and
     writes exactly mib MiB per dispatch, the yardstick for the HBM byte counters."""
@@ -154,6 +233,11 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=2000)
+    ap.add_argument("--serve-steps", type=int, default=200)
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="Poisson arrival rate as a fraction of the measured batched capacity")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE against a copy of known size) and exit")
@@ -182,6 +266,9 @@ def main(argv=None):
         log(f"bench: FP32 form, {a.warmup} warmup + {a.steps} timed steps")
         r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0)
         log(f"bench: FP32 {r32['gcells']:.3f} Gcell/s, {r32['ms_per_step']:.3f} ms/step")
+    serving = None
+    if not a.no_serve:
+        serving = run_serving(a.serve_jobs, a.serve_steps, a.serve_load, 0.05)
     cpu = None
     if not a.no_cpu:
         log(f"bench: CPU oracle baseline, ~{a.cpu_seconds:.0f} s sample")
@@ -248,6 +335,8 @@ def main(argv=None):
             },
             "rel_L1_vs_fp64_after_steps": l1,
         }
+    if serving is not None:
+        out["serving"] = serving
     if cpu is not None:
         out["cpu_baseline"] = {
             "value": round(cpu["value"], 5),

@@ -199,7 +199,7 @@ def serve_batched(specs: list[JobSpec], precision: int = FP32, config: BatchConf
                 # idle: wait for the next arrival
                 dt = pending[0].arrival - clock()
                 if dt > 0:
-                    time.sleep(min(dt, 0.01))
+                    time.sleep(min(dt, 2e-4))
         t_end = time.perf_counter() - t0
     first = min(s.arrival for s in specs) if realtime else 0.0
     res.makespan = t_end - first
@@ -268,7 +268,7 @@ def _time_batch(specs_params, config: BatchConfig, launches: int = 3, warm: int 
 
 
 def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells: int = 12 << 20,
-             cache: str | None = None, lib=None, log=None) -> dict:
+             segs=SEG_CANDIDATES, blocks_per_cu=(0, 3, 2), cache: str | None = None, lib=None, log=None) -> dict:
     """Choose the segment height (rows per work item) per (setup, grid) and the persistent
     workgroups per CU of the batch, by timing batches of identical jobs (enough copies to
     fill the GPU) for each candidate. Results are cached in `cache` (JSON) when given."""
@@ -285,8 +285,8 @@ def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells
             spec = JobSpec(0, 0.0, kind, nx, ny, 1.4, 5e-3, 1, 0)
             copies = max(1, min(256, target_cells // (nx * ny)))
             best, best_rate = 0, -1.0
-            for seg in SEG_CANDIDATES:
-                if seg > (ny + 3) // 4 * 4 and seg != SEG_CANDIDATES[0]:
+            for seg in segs:
+                if seg > (ny + 3) // 4 * 4 and seg != segs[0]:
                     continue
                 p = job_params(spec, precision, seg)
                 cfg = BatchConfig.defaults(precision=precision, max_jobs=max(copies, 1))
@@ -301,7 +301,7 @@ def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells
     sample = make_stream(96, rate=0, seed=99, steps=0, sizes=sizes)
     mix = [(job_params(s, precision, _tuned_seg(out, s)), job_problem(s)) for s in sample]
     best_b, best_rate = 0, -1.0
-    for bpc in (0, 3, 2):
+    for bpc in blocks_per_cu:
         cfg = BatchConfig.defaults(precision=precision, max_jobs=len(mix), blocks_per_cu=bpc)
         rate = _time_batch(mix, cfg, lib=lib)
         out["rates"][f"mix:bpc{bpc}"] = rate
@@ -319,4 +319,15 @@ def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
         with open(cache, "w") as f:
             json.dump(data, f, indent=1)
+    return out
+
+
+def field_errors(q: np.ndarray, ref: np.ndarray) -> dict:
+    """Relative L1 and Linf error per primitive field (rho, u, v, prs)."""
+    out = {}
+    for k, name in enumerate(("rho", "u", "v", "prs")):
+        d = np.abs(q[k] - ref[k])
+        s1 = max(float(np.abs(ref[k]).mean()), 1e-300)
+        si = max(float(np.abs(ref[k]).max()), 1e-300)
+        out[name] = (float(d.mean()) / s1, float(d.max()) / si)
     return out
