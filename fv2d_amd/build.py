@@ -72,14 +72,48 @@ def build(jobs: int = 3, verbose: bool = False, defines=(), out: str = LIB) -> s
     return out
 
 
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libfv2d_host.so")
+HOST_EXE = os.path.join(HERE, "fv2d")
+HOST_SOURCES = ["ini.cpp", "params.cpp", "init.cpp", "h5.cpp", "io.cpp", "host_capi.cpp"]
+HDF5_INCLUDE = os.environ.get("FV2D_HDF5_INCLUDE", "/opt/conda/include")
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-ffp-contract=off", f"-I{HDF5_INCLUDE}"]
+
+
+def build_host(verbose: bool = False) -> tuple[str, str]:
+    """The host side: libfv2d_host.so (.ini reader, parameters, initial conditions, HDF5
+    snapshots; HDF5 is loaded at run time) and the fv2d executable (the time loop over the
+    C ABI, linked against libfv2d_amd.so)."""
+    def run(cmd):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{r.stderr}")
+        if verbose and r.stderr.strip():
+            print(r.stderr, file=sys.stderr)
+
+    tmp = HOST_LIB + ".tmp"
+    run([CXX, *HOST_FLAGS, "-shared", "-o", tmp, *[os.path.join(HOST, s) for s in HOST_SOURCES], "-ldl"])
+    os.replace(tmp, HOST_LIB)
+    tmp = HOST_EXE + ".tmp"
+    run([CXX, *HOST_FLAGS, "-o", tmp, os.path.join(HOST, "main.cpp"), os.path.join(HOST, "driver.cpp"),
+         f"-L{HERE}", "-lfv2d_host", "-lfv2d_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_EXE)
+    return HOST_LIB, HOST_EXE
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=3)
     ap.add_argument("-v", "--verbose", action="store_true")
     ap.add_argument("--define", action="append", default=[], help="NAME=VALUE for a build variant")
     ap.add_argument("--out", default=LIB)
+    ap.add_argument("--host-only", action="store_true", help="build only libfv2d_host.so and fv2d")
     a = ap.parse_args(argv)
-    print(build(a.jobs, a.verbose, tuple(a.define), a.out))
+    if not a.host_only:
+        print(build(a.jobs, a.verbose, tuple(a.define), a.out))
+    if not a.define and a.out == LIB:
+        print(*build_host(a.verbose))
 
 
 if __name__ == "__main__":
