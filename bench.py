@@ -128,7 +128,7 @@ def run_cpu(size: int, seconds: float, q0: np.ndarray):
     return dict(value=size * size * n / el / 1e9, steps=n, seconds=el, threads=lib.orc_get_threads())
 
 
-def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: int = 2024):
+def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: int = 2024, use_graph: bool = True):
     """The served job stream: autotune, batched and one-at-a-time capacity (all jobs queued
     at t = 0), batched latency under Poisson arrivals at `load` x the batched capacity, and
     the FP64 re-run of a seeded subsample as the accuracy check of the FP32 form."""
@@ -136,12 +136,12 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
     from fv2d_amd.capi import FP32, FP64, BatchConfig
 
     t0 = time.perf_counter()
-    tune = S.autotune(FP32, segs=(32, 64), blocks_per_cu=(0,), log=log)
+    tune = S.autotune(FP32, segs=(32, 64), blocks_per_cu=(0,), log=log, use_graph=int(use_graph))
     t_tune = time.perf_counter() - t0
     specs = S.make_stream(n_jobs, rate=0, seed=seed, steps=steps)
     rng = np.random.default_rng(seed + 1)
     check = set(int(i) for i in rng.choice(n_jobs, max(1, int(round(check_frac * n_jobs))), replace=False))
-    cfg = BatchConfig.defaults(precision=FP32, max_jobs=128)
+    cfg = BatchConfig.defaults(precision=FP32, max_jobs=128, use_graph=int(use_graph))
     log(f"bench: serving {n_jobs} jobs x {steps} RK2 steps, batched (all queued at t=0)")
     rb, prim32 = S.serve_batched(specs, FP32, cfg, tuning=tune, keep_prim=check, realtime=False)
     log(f"bench: batched {rb.throughput / 1e9:.2f} Gcell/s, makespan {rb.makespan:.2f} s")
@@ -158,8 +158,9 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
     lat_s = S.fifo_latencies(specs_rt, service)
     log(f"bench: FP64 re-run of {len(check)} jobs")
     sub = [s for s in specs if s.idx in check]
-    r64, prim64 = S.serve_batched(sub, FP64, BatchConfig.defaults(precision=FP64, max_jobs=64), keep_prim=check,
-                                  realtime=False)
+    r64, prim64 = S.serve_batched(sub, FP64, BatchConfig.defaults(precision=FP64, max_jobs=64,
+                                                                  use_graph=int(use_graph)),
+                                  keep_prim=check, realtime=False)
     steps32 = {r.idx: r.nsteps for r in rb.records}
     steps64 = {r.idx: r.nsteps for r in r64.records}
     errs = {name: [] for name in ("rho", "u", "v", "prs")}
@@ -234,6 +235,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (for profiling)")
+    ap.add_argument("--serve-no-graph", action="store_true",
+                    help="batched launches without graph replay (same kernels; for tracing tools)")
     ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--serve-load", type=float, default=0.7,
@@ -255,6 +259,10 @@ def main(argv=None):
     from fv2d_amd import problems as P
 
     capi.load_library()  # fail loudly when the HIP library is missing
+    if a.serve_only:
+        print(json.dumps({"serving": run_serving(a.serve_jobs, a.serve_steps, a.serve_load, 0.05,
+                                                 use_graph=not a.serve_no_graph)}), flush=True)
+        return 0
     log(f"bench: building KH {a.size}^2 initial state")
     q0 = P.kh_ic(P.kh_params(N=a.size))
 

@@ -100,11 +100,13 @@ FV2D_HD StripGeom strip_geom(int Nx, int Ny, int seg) {
 }
 
 // Workgroup id -> (strip, segment). Consecutive workgroup ids are dispatched round-robin
-// over the 8 XCDs; when the count allows, XCD x gets a contiguous run of strips so that
+// over the 8 XCDs (id % 8); XCD x gets the contiguous run of logical items
+// [x*q + min(x, r), ...) of its q or q+1 workgroups (nblocks = 8q + r), so that
 // x-neighbouring strips (which share halo columns) run on the same L2.
 FV2D_HD void strip_of_block(const StripGeom &g, int blk, int &sx, int &sy) {
-  int logical = blk;
-  if (g.nblocks % 8 == 0) logical = (blk % 8) * (g.nblocks / 8) + blk / 8;
+  const int q = g.nblocks >> 3, r = g.nblocks & 7;
+  const int x = blk & 7, k = blk >> 3;
+  const int logical = x * q + (x < r ? x : r) + k;
   sx = logical % g.nsx;
   sy = logical / g.nsx;
 }

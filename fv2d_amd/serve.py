@@ -268,7 +268,8 @@ def _time_batch(specs_params, config: BatchConfig, launches: int = 3, warm: int 
 
 
 def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells: int = 12 << 20,
-             segs=SEG_CANDIDATES, blocks_per_cu=(0, 3, 2), cache: str | None = None, lib=None, log=None) -> dict:
+             segs=SEG_CANDIDATES, blocks_per_cu=(0, 3, 2), cache: str | None = None, lib=None, log=None,
+             use_graph: int = 1) -> dict:
     """Choose the segment height (rows per work item) per (setup, grid) and the persistent
     workgroups per CU of the batch, by timing batches of identical jobs (enough copies to
     fill the GPU) for each candidate. Results are cached in `cache` (JSON) when given."""
@@ -289,7 +290,7 @@ def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells
                 if seg > (ny + 3) // 4 * 4 and seg != segs[0]:
                     continue
                 p = job_params(spec, precision, seg)
-                cfg = BatchConfig.defaults(precision=precision, max_jobs=max(copies, 1))
+                cfg = BatchConfig.defaults(precision=precision, max_jobs=max(copies, 1), use_graph=use_graph)
                 rate = _time_batch([(p, job_problem(spec))] * copies, cfg, lib=lib)
                 out["rates"][f"{KIND_NAMES[kind]}:{nx}x{ny}:seg{seg}"] = rate
                 if rate > best_rate:
@@ -302,7 +303,7 @@ def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells
     mix = [(job_params(s, precision, _tuned_seg(out, s)), job_problem(s)) for s in sample]
     best_b, best_rate = 0, -1.0
     for bpc in blocks_per_cu:
-        cfg = BatchConfig.defaults(precision=precision, max_jobs=len(mix), blocks_per_cu=bpc)
+        cfg = BatchConfig.defaults(precision=precision, max_jobs=len(mix), blocks_per_cu=bpc, use_graph=use_graph)
         rate = _time_batch(mix, cfg, lib=lib)
         out["rates"][f"mix:bpc{bpc}"] = rate
         if rate > best_rate:
