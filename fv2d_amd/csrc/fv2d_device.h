@@ -14,6 +14,14 @@ __device__ inline double lane_shr1(double x) {
 }
 __device__ inline float lane_shr1(float x) { return __int_as_float(lane_shr1_i(__float_as_int(x))); }
 
+// Values that are the same in every lane (control-block fields): read once into scalar
+// registers, so that branches on them stay uniform and they take no VGPRs.
+__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ inline double uni(double x) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(x)));
+}
+
 // Wave64 max of a non-negative double.
 __device__ inline double wave_max(double m) {
 #pragma unroll
@@ -102,8 +110,8 @@ __global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D
 strip_kernel(const KParams kp, const StageIO<S> io, const StripGeom g) {
   __shared__ typename K::Smem sm;
   const StepCtl *ctl = io.ctl;
-  if (!ctl->active) return;  // uniform: the device loop has ended
-  strip_block<K, R, S>(kp, io, g, blockIdx.x, ctl->dt, ctl->dt_dx, ctl->dt_dy, sm);
+  if (!uni(ctl->active)) return;  // the device loop has ended
+  strip_block<K, R, S>(kp, io, g, blockIdx.x, uni(ctl->dt), uni(ctl->dt_dx), uni(ctl->dt_dy), sm);
 }
 
 // Batched stage: a persistent grid pulls (job, item) pairs of one kernel class from its run
@@ -127,7 +135,8 @@ batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restric
     const BatchItem wi = items[base + it];
     const BatchJob &J = jobs[wi.job];
     StepCtl *ctl = J.ctl;
-    if (!ctl->active) continue;
+    if (!uni(ctl->active)) continue;
+    const int fresh = uni(ctl->fresh), nsteps = uni(ctl->nsteps);
     if (!J.rk2 && stage == 2) continue;
     StageIO<S> io;
     io.q0 = (const S *)J.Q0;
@@ -137,7 +146,7 @@ batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restric
     if (J.rk2) {
       if (stage == 1) {
         io.src = (const S *)J.U;
-        io.qsrc = ctl->fresh ? QSRC_Q0 : QSRC_U_RESET;
+        io.qsrc = fresh ? QSRC_Q0 : QSRC_U_RESET;
         io.avg = nullptr;
         io.out = (S *)J.Us;
         io.epilogue = 0;
@@ -149,10 +158,10 @@ batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restric
         io.epilogue = 1;
       }
     } else {
-      const bool odd = ((ctl->nsteps - 1) & 1) != 0;  // nsteps counts the step being taken
+      const bool odd = ((nsteps - 1) & 1) != 0;  // nsteps counts the step being taken
       io.src = (const S *)(odd ? J.Us : J.U);
       io.out = (S *)(odd ? J.U : J.Us);
-      io.qsrc = ctl->fresh ? QSRC_Q0 : QSRC_U_RESET;
+      io.qsrc = fresh ? QSRC_Q0 : QSRC_U_RESET;
       io.avg = nullptr;
       io.epilogue = 1;
     }
@@ -161,7 +170,7 @@ batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restric
     g.nsy = J.nsy;
     g.seg = J.seg;
     g.nblocks = J.nblocks;
-    strip_block<K, R, S>(J.kp, io, g, wi.blk, ctl->dt, ctl->dt_dx, ctl->dt_dy, sm);
+    strip_block<K, R, S>(J.kp, io, g, wi.blk, uni(ctl->dt), uni(ctl->dt_dx), uni(ctl->dt_dy), sm);
     __syncthreads();
   }
 }
