@@ -148,6 +148,13 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
     log("bench: serving one at a time (single-simulation path)")
     rs, _ = S.serve_sequential(specs, FP32)
     log(f"bench: one-at-a-time {rs.throughput / 1e9:.2f} Gcell/s, makespan {rs.makespan:.2f} s")
+    by_size = {}
+    svc = {r.idx: r.finished - r.admitted for r in rs.records}
+    for sp in specs:
+        k = f"{sp.Nx}x{sp.Ny}"
+        c, t = by_size.get(k, (0.0, 0.0))
+        by_size[k] = (c + float(sp.cells) * sp.steps, t + svc[sp.idx])
+    seq_by_size = {k: round(c / t / 1e9, 3) for k, (c, t) in sorted(by_size.items(), key=lambda kv: kv[1][0])}
     cap_jobs = n_jobs / rb.makespan
     rate = load * cap_jobs
     specs_rt = S.with_arrivals(specs, rate, seed=seed + 2)
@@ -183,6 +190,7 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
         "value": round(rb.throughput / 1e9, 4),
         "unit": "Gcell-updates/s",
         "one_at_a_time": round(rs.throughput / 1e9, 4),
+        "one_at_a_time_by_grid": seq_by_size,
         "speedup_vs_one_at_a_time": round(rb.throughput / rs.throughput, 3),
         "makespan_s": {"batched": round(rb.makespan, 3), "one_at_a_time": round(rs.makespan, 3)},
         "gpu_busy_frac": round(rb.gpu_ms / 1e3 / rb.makespan, 3),

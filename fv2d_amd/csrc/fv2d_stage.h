@@ -106,7 +106,8 @@ FV2D_HD StripGeom strip_geom(int Nx, int Ny, int seg) {
 FV2D_HD void strip_of_block(const StripGeom &g, int blk, int &sx, int &sy) {
   const int q = g.nblocks >> 3, r = g.nblocks & 7;
   const int x = blk & 7, k = blk >> 3;
-  const int logical = x * q + (x < r ? x : r) + k;
+  int logical = x * q + (x < r ? x : r) + k;
+  if (g.nblocks < 0) logical = blk;  // work-list items: no dispatch order to follow
   sx = logical % g.nsx;
   sy = logical / g.nsx;
 }

@@ -227,3 +227,66 @@ def test_serve_loop_small_stream(gpu_lib):
     sb = {r.idx: (r.nsteps, r.t, r.sums) for r in rb.records}
     ss = {r.idx: (r.nsteps, r.t, r.sums) for r in rs.records}
     assert sb == ss
+
+
+def _random_jobs(n, seed, prec):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(n):
+        kind = int(rng.integers(0, 3))
+        nx, ny = int(rng.integers(16, 300)), int(rng.integers(8, 200))
+        sp = spec(kind, nx, ny, float(rng.uniform(1.4, 5 / 3)), float(rng.uniform(1e-3, 1e-2)),
+                  int(rng.integers(1, 1 << 40)), steps=int(rng.integers(3, 25)), idx=k)
+        jobs.append(sp)
+    return jobs
+
+
+def _run_random_stream(jobs, prec, lib, seed):
+    """Admit the jobs in a random order at random launch boundaries into a small batch
+    (few slots: admission waits for finished jobs); returns {idx: (q, info)}."""
+    rng = np.random.default_rng(seed)
+    order = list(rng.permutation(len(jobs)))
+    cfg = BatchConfig.defaults(precision=prec, max_jobs=7, steps_per_launch=3, arena_bytes=256 << 20)
+    out, ids = {}, {}
+    with Batch(cfg, lib=lib) as b:
+        while order or ids:
+            for _ in range(int(rng.integers(0, 4))):
+                if not order:
+                    break
+                sp = jobs[order[0]]
+                try:
+                    jid = b.submit(S.job_params(sp, prec), problem=S.job_problem(sp), max_steps=sp.steps)
+                except BatchBusy:
+                    break
+                ids[jid] = order.pop(0)
+            if ids:
+                b.run(1)
+            for jid in b.poll():
+                q, _, inf = b.collect(jid)
+                out[ids.pop(jid)] = (q, inf)
+    return out
+
+
+def test_random_stream_fp32_each_job_equals_alone(gpu_lib):
+    """40 jobs of random setup, grid (16..300 x 8..200), gamma, amplitude, seed and step
+    count, admitted in random order at random launches into a 7-slot batch: every result is
+    bitwise the result of the same job alone in a batch (FP32 form)."""
+    jobs = _random_jobs(40, 123, FP32)
+    got = _run_random_stream(jobs, FP32, gpu_lib, 9)
+    assert sorted(got) == list(range(40))
+    for k, sp in enumerate(jobs):
+        one = run_batch([(S.job_params(sp, FP32), S.job_problem(sp), None, sp.steps)], FP32, gpu_lib)[0]
+        q, inf = got[k]
+        assert inf.nsteps == sp.steps and inf.t == one[2].t
+        assert np.array_equal(q, one[0]), f"job {k} ({S.KIND_NAMES[sp.kind]} {sp.Nx}x{sp.Ny}) differs alone"
+
+
+def test_random_stream_fp64_each_job_equals_single_path(gpu_lib):
+    """The same with FP64 jobs, against the single-simulation path (bitwise)."""
+    jobs = _random_jobs(16, 321, FP64)
+    got = _run_random_stream(jobs, FP64, gpu_lib, 4)
+    for k, sp in enumerate(jobs):
+        qa, _, ta, na = alone(sp, FP64, gpu_lib)
+        q, inf = got[k]
+        assert inf.nsteps == na and inf.t == ta
+        assert np.array_equal(q, qa), f"job {k} ({S.KIND_NAMES[sp.kind]} {sp.Nx}x{sp.Ny}) differs from the single path"
