@@ -72,6 +72,8 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
   r.neg_rho = r.neg_prs = r.nan_cnt = 0;
   r.inv = 0.0;
 
+  RowPrefetch<S> pf;
+  if (K::kPF && jb < je) K::chunk_prefetch(lane, w, 0, i0, jb, kp, io, pf);
   K::pro_load(lane, w, i0, jb, sm, kp, io);
   __syncthreads();
   K::pro_slopes(lane, w, jb, sm);
@@ -81,7 +83,12 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
 
   int xw = 0;
   for (int jc = jb; jc < je; jc += STRIP_ROWS) {
-    K::chunk_load(lane, w, xw, i0, jb, jc, sm, kp, io, r);
+    if (K::kPF) {
+      K::chunk_commit(lane, w, xw, i0, jb, jc, sm, kp, io, pf, r);
+      if (jc + STRIP_ROWS < je) K::chunk_prefetch(lane, w, (xw + 1) & (STRIP_ROWS - 1), i0, jc + STRIP_ROWS, kp, io, pf);
+    } else {
+      K::chunk_load(lane, w, xw, i0, jb, jc, sm, kp, io, r);
+    }
     __syncthreads();
     K::pass1_slopes(lane, w, jb, jc, sm, r);
 #pragma unroll

@@ -112,6 +112,15 @@ FV2D_HD void strip_of_block(const StripGeom &g, int blk, int &sx, int &sy) {
   sy = logical / g.nsx;
 }
 
+// Raw values (U or Q0, storage type) of the next chunk's ring row and halo columns, loaded
+// one chunk ahead so that the load latency overlaps the current chunk's work.
+template <typename S>
+struct RowPrefetch {
+  S row[4];
+  S extra[4];
+  int flags;  // ghost sign flips: bit 0/1 row u/v, bit 2/3 extra u/v
+};
+
 // Per-lane state carried between phases (registers on the GPU).
 template <typename R>
 struct LaneRegs {
@@ -141,6 +150,17 @@ struct StripStage {
   static constexpr int NY = 6;            // y-slope / y-flux ring rows
   static constexpr int NV = 5;            // viscous y-face ring rows
   static constexpr bool kTC = TC, kVISC = VISC;
+#ifndef FV2D_PREFETCH_FP64
+#define FV2D_PREFETCH_FP64 0
+#endif
+  // Register prefetch of the next chunk's rows (FP64: only when it fits the VGPR budget).
+#ifndef FV2D_NO_PREFETCH
+  static constexpr bool kPF = sizeof(S) == 4 || FV2D_PREFETCH_FP64;
+#else
+  static constexpr bool kPF = false;
+#endif
+  // the halo columns too (FP32: 4 more registers; FP64 loads them in chunk_commit)
+  static constexpr bool kPFX = kPF && sizeof(S) == 4;
 
   struct Smem {
     R q[4][NQ][HX];
@@ -158,24 +178,43 @@ struct StripStage {
   static FV2D_HD int yslot(int j, int jb) { return (j - jb + 1) % NY; }
   static FV2D_HD int vslot(int j, int jb) { return (j - jb + 1) % NV; }
 
-  // Primitive state of domain-relative cell (gi, gj), ghosts included.
-  static FV2D_HD void fetch_q(int gi, int gj, const KParams &kp, const StageIO<S> &io, St<R> &q) {
-    bool fx = false, fy = false;
+  // Storage values of domain-relative cell (gi, gj), ghosts included (mapped to their source
+  // cell; fx / fy: the normal velocity changes sign).
+  static FV2D_HD void fetch_raw(int gi, int gj, const KParams &kp, const StageIO<S> &io, S raw[4], bool &fx,
+                                bool &fy) {
+    fx = false;
+    fy = false;
     const int si = (gi >= 0 && gi < kp.Nx) ? gi : map_ghost(gi, kp.Nx, kp.bcx, fx);
     const int sj = (gj >= 0 && gj < kp.Ny) ? gj : map_ghost(gj, kp.Ny, kp.bcy, fy);
     const long long k = (long long)sj * kp.pitch + si;
+    const S *base = (io.qsrc == QSRC_Q0) ? io.q0 : io.src;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) raw[v] = base[v * kp.plane + k];
+  }
+
+  // Primitives from storage values: Q0 as given, else consToPrim (+ reset).
+  static FV2D_HD void raw_to_q(const S raw[4], bool fx, bool fy, const KParams &kp, const StageIO<S> &io,
+                               St<R> &q) {
     if (io.qsrc == QSRC_Q0) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) q.v[v] = R(io.q0[v * kp.plane + k]);
+      for (int v = 0; v < 4; ++v) q.v[v] = R(raw[v]);
     } else {
       St<R> u;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) u.v[v] = R(io.src[v * kp.plane + k]);
+      for (int v = 0; v < 4; ++v) u.v[v] = R(raw[v]);
       q = cons_to_prim(u, R(kp.gm1));
       if (io.qsrc == QSRC_U_RESET) reset_negative(q, R(kp.eps_reset));
     }
     if (fx) q.v[IU] = -q.v[IU];
     if (fy) q.v[IV] = -q.v[IV];
+  }
+
+  // Primitive state of domain-relative cell (gi, gj), ghosts included.
+  static FV2D_HD void fetch_q(int gi, int gj, const KParams &kp, const StageIO<S> &io, St<R> &q) {
+    S raw[4];
+    bool fx, fy;
+    fetch_raw(gi, gj, kp, io, raw, fx, fy);
+    raw_to_q(raw, fx, fy, kp, io, q);
   }
 
   static FV2D_HD void put(Smem &sm, int slot, int col, const St<R> &q) {
@@ -314,6 +353,36 @@ struct StripStage {
                                  const StageIO<S> &io, LaneRegs<R> &r) {
     load_row(lane, w, jc + 2 + w, i0, jb, sm, kp, io);
     if (w == xw) load_extra(lane, jc + 2, 4, i0, jb, sm, kp, io);
+    load_own(lane, jc + w, i0, kp, io, r);
+  }
+
+  // Prefetched form of chunk_load's ring rows: the loads of chunk jn's rows jn+2+w (and of
+  // its halo columns by wave xwn) are issued one chunk ahead; chunk_commit converts them.
+  static FV2D_HD void chunk_prefetch(int lane, int w, int xwn, int i0, int jn, const KParams &kp,
+                                     const StageIO<S> &io, RowPrefetch<S> &p) {
+    bool fx, fy;
+    fetch_raw(i0 - 2 + lane, jn + 2 + w, kp, io, p.row, fx, fy);
+    p.flags = (fx ? 1 : 0) | (fy ? 2 : 0);
+    if (kPFX && w == xwn && lane < 8) {
+      fetch_raw(i0 - 2 + STRIP_W + (lane & 1), jn + 2 + (lane >> 1), kp, io, p.extra, fx, fy);
+      p.flags |= (fx ? 4 : 0) | (fy ? 8 : 0);
+    }
+  }
+  static FV2D_HD void chunk_commit(int lane, int w, int xw, int i0, int jb, int jc, Smem &sm, const KParams &kp,
+                                   const StageIO<S> &io, const RowPrefetch<S> &p, LaneRegs<R> &r) {
+    St<R> q;
+    raw_to_q(p.row, (p.flags & 1) != 0, (p.flags & 2) != 0, kp, io, q);
+    put(sm, qslot(jc + 2 + w, jb), lane, q);
+    if (w == xw) {
+      if (kPFX) {
+        if (lane < 8) {
+          raw_to_q(p.extra, (p.flags & 4) != 0, (p.flags & 8) != 0, kp, io, q);
+          put(sm, qslot(jc + 2 + (lane >> 1), jb), STRIP_W + (lane & 1), q);
+        }
+      } else {
+        load_extra(lane, jc + 2, 4, i0, jb, sm, kp, io);
+      }
+    }
     load_own(lane, jc + w, i0, kp, io, r);
   }
 

@@ -77,12 +77,21 @@ void run_stage(Sim &s, const StageIO<S> &io) {
     (void)r;                                                                                                           \
     stmt;                                                                                                              \
   }
+    std::vector<RowPrefetch<S>> pfs(K::NT);
+    for (auto &p : pfs) memset(&p, 0xff, sizeof p);  // garbage until loaded
+    if (K::kPF && jb < je) LANES(K::chunk_prefetch(lane, w, 0, i0, jb, kp, io, pfs[t]));
     LANES(K::pro_load(lane, w, i0, jb, *sm, kp, io));
     LANES(K::pro_slopes(lane, w, jb, *sm));
     LANES(K::pro_flux(lane, w, jb, *sm, kp));
     int xw = 0;
     for (int jc = jb; jc < je; jc += STRIP_ROWS) {
-      LANES(K::chunk_load(lane, w, xw, i0, jb, jc, *sm, kp, io, r));
+      if (K::kPF) {
+        LANES(K::chunk_commit(lane, w, xw, i0, jb, jc, *sm, kp, io, pfs[t], r));
+        if (jc + STRIP_ROWS < je)
+          LANES(K::chunk_prefetch(lane, w, (xw + 1) & (STRIP_ROWS - 1), i0, jc + STRIP_ROWS, kp, io, pfs[t]));
+      } else {
+        LANES(K::chunk_load(lane, w, xw, i0, jb, jc, *sm, kp, io, r));
+      }
       LANES(K::pass1_slopes(lane, w, jb, jc, *sm, r));
       shift(regs, [](LaneRegs<R> &d, const LaneRegs<R> *s_) {
         for (int v = 0; v < 4; ++v) d.sxl[v] = s_ ? s_->sx[v] : R(0);

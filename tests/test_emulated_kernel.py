@@ -76,6 +76,7 @@ def test_sanitized_bounds(oracle_lib):
         "    if n %% 9: continue\n"
         "    p, q0 = T.make(n, *c)\n"
         "    T.compare(p, q0, lib, olib, steps=2)\n"
+        "T.fp32_close(lib, olib, N=24, steps=4)\n"
         "print('ok')\n" % ([os.path.dirname(__file__), os.path.dirname(os.path.dirname(__file__)),
                            os.path.join(os.path.dirname(__file__), "emu")],))
     import subprocess as sp
@@ -88,12 +89,16 @@ def test_sanitized_bounds(oracle_lib):
 
 def test_fp32_form_close_to_fp64(emu_lib, oracle_lib):
     """FP32 form (emulated): KH 64^2, 20 steps, relative L1 of every field <= 1e-5."""
-    p, q0, _ = P.workload("kh", N=64)
+    fp32_close(emu_lib, oracle_lib)
+
+
+def fp32_close(emu_lib, oracle_lib, N=64, steps=20):
+    p, q0, _ = P.workload("kh", N=N)
     o = Oracle(p, oracle_lib)
     e = EMU.Emu(p.copy(precision=FP32), emu_lib)
     o.set_prim(q0)
     e.set_prim(q0)
-    for _ in range(20):
+    for _ in range(steps):
         dto, _ = o.compute_dt()
         dte, _ = e.compute_dt()
         assert abs(dte - dto) <= 1e-5 * dto
