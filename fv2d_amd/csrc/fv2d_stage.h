@@ -27,6 +27,8 @@
 // and DPP lane shifts in between; the host emulation runs them lane by lane.
 #pragma once
 
+#include <type_traits>
+
 #include "fv2d_math.h"
 
 namespace fv2d {
@@ -465,24 +467,28 @@ struct StripStage {
     }
   }
 
-  // U += ((FL - FR) * dt) / d (Update.h:158-159; States.h:49-101 fix the order). In the
-  // FP32 form the flux difference is applied in FP64 with a precomputed dt / d.
-  static FV2D_HD void add_flux(double acc[4], const R FL[4], const R FR[4], double dt, double dt_d, double d,
-                               double rd, int fastdiv) {
+  // Accumulator of the update. FP64: U itself, every term added in the reference's order
+  // (Update.h:158-166, States.h:49-101). FP32 form: the increment dt*dF of the step in
+  // FP32, added to U in FP64 once at the end.
+  typedef typename std::conditional<sizeof(R) == 8, double, float>::type A;
+
+  // U += ((FL - FR) * dt) / d (Update.h:158-159); FP32 form: (FL - FR) * (dt / d).
+  static FV2D_HD void add_flux(A acc[4], const R FL[4], const R FR[4], double dt, double dt_d, double d, double rd,
+                               int fastdiv) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       if (sizeof(R) == 8)
-        acc[v] = acc[v] + Ops<double>::div_c((double)(FL[v] - FR[v]) * dt, d, rd, fastdiv);
+        acc[v] = acc[v] + A(Ops<double>::div_c((double)(FL[v] - FR[v]) * dt, d, rd, fastdiv));
       else
-        acc[v] = acc[v] + ((double)FL[v] - (double)FR[v]) * dt_d;
+        acc[v] = acc[v] + A(FL[v] - FR[v]) * A(dt_d);
     }
   }
 
   // Gravity source (Update.h:161-166): always into the y momentum; energy from the mass
   // fluxes of the sweep.
-  static FV2D_HD void add_gravity(double acc[4], R rho, R FL0, R FR0, double dt, double g) {
-    acc[IV] = acc[IV] + dt * (double)rho * g;
-    acc[IE] = acc[IE] + dt * 0.5 * ((double)FL0 + (double)FR0) * g;
+  static FV2D_HD void add_gravity(A acc[4], R rho, R FL0, R FR0, double dt, double g) {
+    acc[IV] = acc[IV] + A(dt) * A(rho) * A(g);
+    acc[IE] = acc[IE] + A(dt) * A(0.5) * (A(FL0) + A(FR0)) * A(g);
   }
 
   // Update part 2: the cell (lane, jc+w) (needs fxl, tcxl, vxl from lane-1).
@@ -493,9 +499,9 @@ struct StripStage {
     if (lane < 2 || gi >= kp.Nx || j >= je) return;
     const R rho = Q(sm, IR, j, lane, jb);
 
-    double acc[4];
+    A acc[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] = r.u[v];
+    for (int v = 0; v < 4; ++v) acc[v] = (sizeof(R) == 8) ? A(r.u[v]) : A(0);
 
     double gX = 0.0, gY = 0.0;
     if (XG) {
@@ -561,7 +567,7 @@ struct StripStage {
         else if (kp.bctc_ymax == BCTC_FIXED_GRADIENT)
           FR = kf * R(kp.bctc_ymax_value);
       }
-      acc[IE] = acc[IE] + (dt_dx * (double)(FR - FL) + dt_dy * (double)(FD - FU));
+      acc[IE] = acc[IE] + (A(dt_dx) * A(FR - FL) + A(dt_dy) * A(FD - FU));
     }
 
     // viscosity (Viscosity.h:33-117): every face term is computed once (x faces by the lane
@@ -576,23 +582,27 @@ struct StripStage {
       const R fyv = (R(0) + (-sm.vy[VISC ? 1 : 0][VISC ? lo : 0][lane])) + sm.vy[VISC ? 1 : 0][VISC ? hi : 0][lane];
       const R fye = (R(0) + (-sm.vy[VISC ? 2 : 0][VISC ? lo : 0][lane])) + sm.vy[VISC ? 2 : 0][VISC ? hi : 0][lane];
       // U += dt * (vf_x + vf_y) (Viscosity.h:116); the density components are zero
-      acc[IR] = acc[IR] + (double)(R(0) + R(0)) * dt;
-      acc[IU] = acc[IU] + (double)(fxu + fyu) * dt;
-      acc[IV] = acc[IV] + (double)(fxv + fyv) * dt;
-      acc[IE] = acc[IE] + (double)(fxe + fye) * dt;
+      acc[IR] = acc[IR] + A(R(0) + R(0)) * A(dt);
+      acc[IU] = acc[IU] + A(fxu + fyu) * A(dt);
+      acc[IV] = acc[IV] + A(fxv + fyv) * A(dt);
+      acc[IE] = acc[IE] + A(fxe + fye) * A(dt);
     }
+
+    double un64[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) un64[v] = (sizeof(R) == 8) ? double(acc[v]) : r.u[v] + double(acc[v]);
 
     // SSP-RK2 average (Update.h:214-220)
     if (io.avg) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] = 0.5 * (r.u0[v] + acc[v]);
+      for (int v = 0; v < 4; ++v) un64[v] = 0.5 * (r.u0[v] + un64[v]);
     }
 
     const long long k = (long long)j * kp.pitch + gi;
     St<R> un;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const S s = S(acc[v]);
+      const S s = S(un64[v]);
       io.out[v * kp.plane + k] = s;
       un.v[v] = R(s);
     }
