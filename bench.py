@@ -207,6 +207,34 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
     }
 
 
+def fp32_errors_all_configs():
+    """FP32 form against the FP64 form (bitwise the oracle's arithmetic) at every benchmark
+    configuration, full size: relative L1 and Linf error per primitive field."""
+    from fv2d_amd import problems as P
+    from fv2d_amd.capi import FP32, FP64
+    from fv2d_amd.solver import Solver
+
+    out = {}
+    for name in ("sod", "sedov", "kh", "rt"):
+        res = {}
+        for prec in (FP64, FP32):
+            p, q0, r = P.workload(name)
+            with Solver(p.copy(precision=prec)) as s:
+                s.set_prim(q0)
+                _, _, _, n = s.advance(tend=r["tend"], max_steps=r["steps"] or 0, record_dts=False)
+                res[prec] = (s.get_prim(), n)
+        a, b = res[FP64][0], res[FP32][0]
+        e = {}
+        for k, f in enumerate(("rho", "u", "v", "prs")):
+            d = np.abs(b[k] - a[k])
+            e[f] = {"L1": float(d.mean() / max(np.abs(a[k]).mean(), 1e-300)),
+                    "Linf": float(d.max() / max(np.abs(a[k]).max(), 1e-300))}
+        p, _, _ = P.workload(name)
+        out[name] = {"grid": [p.Nx, p.Ny], "steps_fp64": res[FP64][1], "steps_fp32": res[FP32][1], "rel_error": e}
+        log(f"bench: FP32 vs FP64 {name}: L1(rho) {e['rho']['L1']:.2e}")
+    return out
+
+
 def calibrate_copy(mib: int, reps: int = 4) -> int:
     """hipMemcpy device-to-device of `mib` MiB, `reps` times: a copy kernel that reads and
     writes exactly mib MiB per dispatch, the yardstick for the HBM byte counters."""
@@ -282,6 +310,9 @@ def main(argv=None):
         log(f"bench: FP32 form, {a.warmup} warmup + {a.steps} timed steps")
         r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0)
         log(f"bench: FP32 {r32['gcells']:.3f} Gcell/s, {r32['ms_per_step']:.3f} ms/step")
+    fp32_err = None
+    if not a.no_fp32 and a.size == 4096:
+        fp32_err = fp32_errors_all_configs()
     serving = None
     if not a.no_serve:
         serving = run_serving(a.serve_jobs, a.serve_steps, a.serve_load, 0.05)
@@ -351,6 +382,8 @@ def main(argv=None):
             },
             "rel_L1_vs_fp64_after_steps": l1,
         }
+        if fp32_err is not None:
+            out["fp32_form"]["error_vs_fp64_every_config"] = fp32_err
     if serving is not None:
         out["serving"] = serving
     if cpu is not None:
