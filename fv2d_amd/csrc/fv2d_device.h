@@ -176,11 +176,7 @@ batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restric
     g.nsx = J.nsx;
     g.nsy = J.nsy;
     g.seg = J.seg;
-#ifdef FV2D_BATCH_NO_REMAP
-    g.nblocks = -J.nblocks;
-#else
     g.nblocks = J.nblocks;
-#endif
     strip_block<K, R, S>(J.kp, io, g, wi.blk, uni(ctl->dt), uni(ctl->dt_dx), uni(ctl->dt_dy), sm);
     __syncthreads();
   }

@@ -108,8 +108,7 @@ FV2D_HD StripGeom strip_geom(int Nx, int Ny, int seg) {
 FV2D_HD void strip_of_block(const StripGeom &g, int blk, int &sx, int &sy) {
   const int q = g.nblocks >> 3, r = g.nblocks & 7;
   const int x = blk & 7, k = blk >> 3;
-  int logical = x * q + (x < r ? x : r) + k;
-  if (g.nblocks < 0) logical = blk;  // work-list items: no dispatch order to follow
+  const int logical = x * q + (x < r ? x : r) + k;
   sx = logical % g.nsx;
   sy = logical / g.nsx;
 }
@@ -156,11 +155,7 @@ struct StripStage {
 #define FV2D_PREFETCH_FP64 0
 #endif
   // Register prefetch of the next chunk's rows (FP64: only when it fits the VGPR budget).
-#ifndef FV2D_NO_PREFETCH
   static constexpr bool kPF = sizeof(S) == 4 || FV2D_PREFETCH_FP64;
-#else
-  static constexpr bool kPF = false;
-#endif
   // the halo columns too (FP32: 4 more registers; FP64 loads them in chunk_commit)
   static constexpr bool kPFX = kPF && sizeof(S) == 4;
 
