@@ -173,12 +173,14 @@ def serve_batched(specs: list[JobSpec], precision: int = FP32, config: BatchConf
                 try:
                     jid = b.submit(job_params(s, precision, _tuned_seg(tuning, s)), problem=job_problem(s),
                                    max_steps=s.steps)
-                except BatchBusy:
+                except BatchBusy as e:
+                    if not running and not to_collect:
+                        raise RuntimeError(f"job {s.idx} ({s.Nx}x{s.Ny}) does not fit an empty batch: {e}") from e
                     break
                 queue.popleft()
                 running[jid] = (s, time.perf_counter() - t0)
             launched = False
-            if any(True for _ in running) and len(running) > len(to_collect):
+            if len(running) > len(to_collect):
                 b.launch(1)
                 launched = True
             for jid in to_collect:  # overlaps the launch just queued

@@ -7,7 +7,7 @@
  *
  * The oracle keeps the reference's data model: full arrays with Ng ghost layers
  * (Ntx x Nty cells), Q and U kept separately, slopes in their own arrays, and one pass per
- * reference kernel, in the reference order (SURVEY call stack main.cpp:62-84). Layout is
+ * reference kernel, in the reference order (the time loop of main.cpp:62-84). Layout is
  * SoA [var][j][i] with x fastest.
  */
 #ifndef FV2D_ORACLE_H
