@@ -11,10 +11,12 @@
 //  - work list: pack_items_kernel lists (slot, strip segment) items per kernel class
 //    (reconstruction, Riemann solver, conduction, viscosity, wall/gravity variant); the
 //    persistent batch_stage_kernel of each class pulls items from its run of the list.
-//  - launches: steps_per_launch steps (reduce, then stage 1 of every class, then stage 2 of
-//    every class) are captured once as a HIP graph and replayed; the graph reads the job
+//  - launches: each kernel class runs its own chain of steps_per_launch steps (reduce, stage
+//    1, stage 2) on a stream of its own, forked from and joined back into the batch stream,
+//    so that the drain of one class's persistent kernel overlaps the others' work. The
+//    whole launch is captured once as a HIP graph and replayed; the graph reads the job
 //    table and the work list from device memory, so it stays valid while jobs come and go.
-//    It holds no memset nodes: the work-list counters are cleared by the reduce kernel.
+//    It holds no memset nodes: the work-list counters are cleared by the reduce kernels.
 //  - results: read back on a second stream, so collecting a finished job overlaps the
 //    launches of the others.
 #include <hip/hip_runtime.h>
@@ -42,13 +44,13 @@ constexpr size_t ARENA_ALIGN = 256;
 
 // ------------------------------------------------------------------ device kernels
 
-// One workgroup per slot: reduce the job's dt partials and run its loop control (the
-// single-simulation reduce_kernel in advance mode). Workgroup 0 also clears the work-list
-// counters that this step's stage kernels start from.
-__global__ void __launch_bounds__(256) batch_reduce_kernel(const BatchJob *jobs, int *counters, int ncounters) {
-  if (blockIdx.x == 0 && (int)threadIdx.x < ncounters) counters[threadIdx.x] = 0;
+// One workgroup per slot: reduce the dt partials of each running job of class `cls` and run
+// its loop control (the single-simulation reduce_kernel in advance mode). Workgroup 0 also
+// clears the class's work-list counters that this step's stage kernels start from.
+__global__ void __launch_bounds__(256) batch_reduce_kernel(const BatchJob *jobs, int *counters, int cls) {
+  if (blockIdx.x == 0 && threadIdx.x < 2) counters[2 * cls + threadIdx.x] = 0;
   const BatchJob &J = jobs[blockIdx.x];
-  if (!J.used) return;
+  if (!J.used || J.cls != cls) return;
   StepCtl *ctl = J.ctl;
   if (!ctl->active) return;  // uniform over the workgroup
   double m = 0.0;
@@ -202,7 +204,10 @@ struct fv2d_batch {
   std::map<int64_t, int> by_id;
   std::deque<int64_t> done;
   std::vector<StageClass> classes;
-  std::vector<int> grid;  // persistent workgroups per class
+  std::vector<int> grid;                // persistent workgroups per class
+  std::vector<hipStream_t> cls_stream;  // per class: its chain of steps
+  std::vector<hipEvent_t> cls_join;     // per class: end of its chain in a launch
+  hipEvent_t ev_fork = nullptr;
   int total_items = 0;
   int64_t next_id = 1;
   bool dirty = true;
@@ -266,31 +271,50 @@ static int class_index(fv2d_batch *b, const StageClass &c) {
   for (size_t i = 0; i < b->classes.size(); ++i)
     if (b->classes[i] == c) return (int)i;
   if ((int)b->classes.size() >= MAX_CLASSES) return -1;
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return -1;
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(st);
+    return -1;
+  }
   const int occ = (b->cfg.precision == FV2D_FP32) ? batch_occupancy_fp32(c) : batch_occupancy_fp64(c);
   int per_cu = occ;
   if (b->cfg.blocks_per_cu > 0 && b->cfg.blocks_per_cu < occ) per_cu = b->cfg.blocks_per_cu;
   b->classes.push_back(c);
   b->grid.push_back(per_cu * b->ncu);
+  b->cls_stream.push_back(st);
+  b->cls_join.push_back(ev);
   return (int)b->classes.size() - 1;
 }
 
-// One step of every job: reduce + loop control, then the stages of every class.
-static hipError_t enqueue_batch_step(fv2d_batch *b) {
-  hipLaunchKernelGGL(batch_reduce_kernel, dim3(b->cfg.max_jobs), dim3(256), 0, b->stream, b->d_jobs, b->d_counters,
-                     2 * MAX_CLASSES);
+// One step of the jobs of class c on stream st: reduce + loop control, stage 1, stage 2.
+static hipError_t enqueue_class_step(fv2d_batch *b, int c, hipStream_t st) {
+  hipLaunchKernelGGL(batch_reduce_kernel, dim3(b->cfg.max_jobs), dim3(256), 0, st, b->d_jobs, b->d_counters, c);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool f32 = b->cfg.precision == FV2D_FP32;
   for (int stage = 1; stage <= 2; ++stage) {
-    for (size_t c = 0; c < b->classes.size(); ++c) {
-      int *counter = b->d_counters + 2 * c + (stage - 1);
-      const int *range = b->d_ranges + 2 * c;
-      e = f32 ? launch_batch_fp32(b->classes[c], b->d_jobs, b->d_items, range, counter, stage, b->grid[c], b->stream)
-              : launch_batch_fp64(b->classes[c], b->d_jobs, b->d_items, range, counter, stage, b->grid[c], b->stream);
-      if (e != hipSuccess) return e;
-    }
+    int *counter = b->d_counters + 2 * c + (stage - 1);
+    const int *range = b->d_ranges + 2 * c;
+    e = f32 ? launch_batch_fp32(b->classes[c], b->d_jobs, b->d_items, range, counter, stage, b->grid[c], st)
+            : launch_batch_fp64(b->classes[c], b->d_jobs, b->d_items, range, counter, stage, b->grid[c], st);
+    if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+// One launch: the class chains forked from the batch stream and joined back into it.
+static hipError_t enqueue_launch(fv2d_batch *b) {
+  hipError_t e = hipEventRecord(b->ev_fork, b->stream);
+  for (size_t c = 0; c < b->classes.size() && e == hipSuccess; ++c) {
+    hipStream_t st = b->cls_stream[c];
+    e = hipStreamWaitEvent(st, b->ev_fork, 0);
+    for (int k = 0; k < b->cfg.steps_per_launch && e == hipSuccess; ++k) e = enqueue_class_step(b, (int)c, st);
+    if (e == hipSuccess) e = hipEventRecord(b->cls_join[c], st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->cls_join[c], 0);
+  }
+  return e;
 }
 
 static int ensure_graph(fv2d_batch *b) {
@@ -304,8 +328,7 @@ static int ensure_graph(fv2d_batch *b) {
     b->graph = nullptr;
   }
   BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipSuccess;
-  for (int k = 0; k < b->cfg.steps_per_launch && e == hipSuccess; ++k) e = enqueue_batch_step(b);
+  const hipError_t e = enqueue_launch(b);
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
   if (e != hipSuccess) BFAIL(b, FV2D_E_HIP, "graph capture: %s", hipGetErrorString(e));
@@ -361,9 +384,12 @@ void fv2d_batch_destroy(fv2d_batch *b) {
   if (b->io) (void)hipStreamSynchronize(b->io);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  hipEvent_t evs[] = {b->ev_start, b->ev_end, b->ev_ctl, b->ev_upload};
+  for (hipStream_t st : b->cls_stream) (void)hipStreamSynchronize(st);
+  hipEvent_t evs[] = {b->ev_start, b->ev_end, b->ev_ctl, b->ev_upload, b->ev_fork};
   for (hipEvent_t e : evs)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->cls_join) (void)hipEventDestroy(e);
+  for (hipStream_t st : b->cls_stream) (void)hipStreamDestroy(st);
   void *bufs[] = {b->arena, b->d_jobs, b->d_ctl, b->d_items, b->d_ranges, b->d_counters,
                   b->d_upload, b->d_out, b->d_rowsum, b->d_sums};
   for (void *x : bufs)
@@ -442,6 +468,7 @@ int fv2d_batch_create(const fv2d_batch_config *c, fv2d_batch **out) {
   hipEvent_t *evs[] = {&b->ev_start, &b->ev_end, &b->ev_ctl, &b->ev_upload};
   for (hipEvent_t *ev : evs)
     if ((e = hipEventCreate(ev)) != hipSuccess) return fail("event", e);
+  if ((e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
   if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail("init", e);
   *out = b;
   return FV2D_OK;
@@ -469,7 +496,9 @@ int fv2d_batch_submit(fv2d_batch *b, const fv2d_params *p, const fv2d_problem *p
   if (prob && problem_args(*p, J.kp, *prob, pa))
     BFAIL(b, FV2D_E_INVALID, "fv2d_batch_submit: unknown problem kind %d or bad amplitude", prob->kind);
   const int cls = class_index(b, class_of(*p, J.kp));
-  if (cls < 0) BFAIL(b, FV2D_E_BUSY, "fv2d_batch_submit: more than %d kernel classes in one batch", MAX_CLASSES);
+  if (cls < 0)
+    BFAIL(b, FV2D_E_BUSY, "fv2d_batch_submit: no room for another kernel class (%d at most) or its stream",
+          MAX_CLASSES);
   int slot = -1;
   for (int s = 0; s < b->cfg.max_jobs; ++s)
     if (b->slots[s].status == FV2D_JOB_FREE) {
@@ -596,7 +625,7 @@ int fv2d_batch_launch(fv2d_batch *b, int32_t launches) {
     if (b->cfg.use_graph) {
       BHIP(b, hipGraphLaunch(b->gexec, b->stream));
     } else {
-      for (int s = 0; s < b->cfg.steps_per_launch; ++s) BHIP(b, enqueue_batch_step(b));
+      BHIP(b, enqueue_launch(b));
     }
   }
   BHIP(b, hipEventRecord(b->ev_end, b->stream));
