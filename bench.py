@@ -141,7 +141,7 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
     specs = S.make_stream(n_jobs, rate=0, seed=seed, steps=steps)
     rng = np.random.default_rng(seed + 1)
     check = set(int(i) for i in rng.choice(n_jobs, max(1, int(round(check_frac * n_jobs))), replace=False))
-    cfg = BatchConfig.defaults(precision=FP32, max_jobs=128, use_graph=int(use_graph))
+    cfg = BatchConfig.defaults(precision=FP32, max_jobs=256, use_graph=int(use_graph))
     log(f"bench: serving {n_jobs} jobs x {steps} RK2 steps, batched (all queued at t=0)")
     rb, prim32 = S.serve_batched(specs, FP32, cfg, tuning=tune, keep_prim=check, realtime=False)
     log(f"bench: batched {rb.throughput / 1e9:.2f} Gcell/s, makespan {rb.makespan:.2f} s")
@@ -203,7 +203,7 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
                                              "under the same arrivals"},
         "latency_all_queued_ms": {k: round(v * 1e3, 2) for k, v in rb.latency_percentiles().items()},
         "fp64_check": fp64_check,
-        "autotune": {"seconds": round(t_tune, 1), "seg": tune["seg"]},
+        "autotune": {"seconds": round(t_tune, 1), "seg_choice": tune.get("seg_choice"), "seg": tune["seg"]},
     }
 
 

@@ -270,11 +270,13 @@ def _time_batch(specs_params, config: BatchConfig, launches: int = 3, warm: int 
 
 
 def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells: int = 12 << 20,
-             segs=SEG_CANDIDATES, blocks_per_cu=(0, 3, 2), cache: str | None = None, lib=None, log=None,
-             use_graph: int = 1) -> dict:
-    """Choose the segment height (rows per work item) per (setup, grid) and the persistent
-    workgroups per CU of the batch, by timing batches of identical jobs (enough copies to
-    fill the GPU) for each candidate. Results are cached in `cache` (JSON) when given."""
+             segs=SEG_CANDIDATES, blocks_per_cu=(0, 3, 2), mix_segs=(64,), cache: str | None = None,
+             lib=None, log=None, use_graph: int = 1) -> dict:
+    """Choose the segment height (rows per work item: the tile height of the stage kernel)
+    per (setup, grid) and the persistent workgroups per CU of the batch: first by timing
+    batches of identical jobs (enough copies to fill the GPU) for each candidate height,
+    then on a mixed sample of the stream, where the per-size table competes with uniform
+    heights (`mix_segs`) and the workgroup counts. Results are cached in `cache` (JSON)."""
     lib = lib or capi.load_library()
     key = f"prec{precision}"
     if cache and os.path.exists(cache):
@@ -300,13 +302,37 @@ def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells
             out["seg"][f"{KIND_NAMES[kind]}:{nx}x{ny}"] = best
             if log:
                 log(f"autotune {KIND_NAMES[kind]} {nx}x{ny}: seg {best} ({best_rate / 1e9:.2f} Gcell/s)")
-    # persistent workgroups per CU on a mixed sample of the stream
+    # On a mixed sample of the stream the classes run concurrently, which favours other
+    # segment heights than batches of one class: the per-size table and uniform heights
+    # compete there, then the persistent workgroups per CU. Heights above 64 rows were
+    # measured to add ~2% throughput but ~50% p50 latency (every job of a class waits for
+    # the slowest work item of each step), so they are not candidates by default.
     sample = make_stream(96, rate=0, seed=99, steps=0, sizes=sizes)
-    mix = [(job_params(s, precision, _tuned_seg(out, s)), job_problem(s)) for s in sample]
-    best_b, best_rate = 0, -1.0
-    for bpc in blocks_per_cu:
+
+    def mix_rate(table, bpc):
+        mix = [(job_params(s, precision, int(table.get(f"{KIND_NAMES[s.kind]}:{s.Nx}x{s.Ny}", 0))), job_problem(s))
+               for s in sample]
         cfg = BatchConfig.defaults(precision=precision, max_jobs=len(mix), blocks_per_cu=bpc, use_graph=use_graph)
-        rate = _time_batch(mix, cfg, lib=lib)
+        return _time_batch(mix, cfg, lib=lib)
+
+    tables = {"per-size": dict(out["seg"])}
+    for seg in mix_segs:
+        tables[f"uniform-{seg}"] = {k: seg for k in out["seg"]}
+    best_t, best_rate = "per-size", -1.0
+    for name, table in tables.items():
+        rate = mix_rate(table, 0)
+        out["rates"][f"mix:{name}"] = rate
+        if rate > best_rate:
+            best_t, best_rate = name, rate
+    out["seg"] = tables[best_t]
+    out["seg_choice"] = best_t
+    if log:
+        log(f"autotune mix: segment table {best_t} ({best_rate / 1e9:.2f} Gcell/s)")
+    best_b = 0
+    for bpc in blocks_per_cu:
+        if bpc == 0:
+            continue
+        rate = mix_rate(out["seg"], bpc)
         out["rates"][f"mix:bpc{bpc}"] = rate
         if rate > best_rate:
             best_b, best_rate = bpc, rate
