@@ -204,6 +204,8 @@ class Batch:
 
     def collect(self, jid: int, prim: bool = True):
         """(q or None, sums, info) of a finished job; frees its slot."""
+        if jid not in self.shapes:
+            raise Fv2dError(capi.FV2D_E_INVALID, f"no job {jid} in this batch")
         nx, ny = self.shapes[jid]
         q = np.empty((4, ny, nx)) if prim else None
         sums = np.zeros(4)
