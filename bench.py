@@ -10,6 +10,8 @@ synthetic seeded initial state, all state resident in HBM. One step = one full R
 (dt reduction + two fused stage launches). The FP64 parity form gives `value`
 (Gcell-updates/s = Nx*Ny*K / time of K steps, HIP events on the solver stream, device
 synchronised on both sides); the FP32 form is timed the same way and reported beside it.
+The K steps run as --blocks consecutive device loops with an event between them:
+`repetitions` gives the median, minimum and maximum rate of the blocks.
 
 roofline: the dominant kernel is the fused stage kernel (two launches per step). Its
 algorithmic HBM bytes per cell are 160 B per FP64 RK2 step (stage 1 reads U^n and writes
@@ -26,7 +28,8 @@ form) through the batch engine with continuous batching, against the same jobs o
 time through the single-simulation path: aggregate Gcell-updates/s of each (all jobs
 queued at t = 0), p50/p99 latency (arrival to result) under seeded Poisson arrivals at
 --serve-load x the batched capacity, and a 5% seeded subsample re-run in FP64 with the
-FP32 form's error against it.
+FP32 form's error against it. A seeded 10% subsample also runs one job at a time with the
+reference's loop structure (dt read back to the host, one blocking step per RK2 step).
 
 cpu_baseline: the CPU oracle (a plain-C FP64 restatement of the reference path with the
 reference's pass structure, OpenMP) timed on the host cores on a bounded sample of the
@@ -66,7 +69,7 @@ def read_traffic(prec: str, size: int):
     return float(m.group(1)) if m else None
 
 
-def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray):
+def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, blocks: int = 5):
     from fv2d_amd import hiprt
     from fv2d_amd import problems as P
     from fv2d_amd.capi import FP32, FP64
@@ -80,12 +83,21 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray):
     stream = s.lib.fv2d_get_stream(s.h)
     hiprt.synchronize()
     s.lib.fv2d_profile(s.h, 1)
-    e0, e1 = hiprt.Event(), hiprt.Event()
-    e0.record(stream)
-    _, dts, counts, n = s.advance(max_steps=steps)
-    e1.record(stream)
+    # the K timed steps run as `blocks` consecutive device loops with an event between them:
+    # the total gives `value`, the blocks give the median / spread of repetitions
+    nb = max(1, min(blocks, steps))
+    bounds = [steps * k // nb for k in range(nb + 1)]
+    ev = [hiprt.Event() for _ in range(nb + 1)]
+    ev[0].record(stream)
+    t, n = 0.0, 0
+    for k in range(nb):
+        t, _, counts, taken = s.advance(max_steps=bounds[k + 1] - bounds[k], t0=t, record_dts=False)
+        n += taken
+        ev[k + 1].record(stream)
     hiprt.synchronize()
-    ms = e0.elapsed_ms(e1)
+    ms = ev[0].elapsed_ms(ev[-1])
+    block_rates = [p.Nx * p.Ny * (bounds[k + 1] - bounds[k]) / (ev[k].elapsed_ms(ev[k + 1]) * 1e-3) / 1e9
+                   for k in range(nb)]
     import ctypes as C
 
     st_ms = C.c_double()
@@ -103,7 +115,9 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray):
     s.close()
     return dict(gcells=gcells, ms_per_step=ms / steps, mean_launch_ms=mean_launch_s * 1e3, launches=int(st_n.value),
                 achieved=achieved, bytes_per_launch=bpl * cells, counts=counts, q=q, sums=sums,
-                dt_first=float(dts[0]) if len(dts) else None)
+                blocks={"n": nb, "steps": [bounds[k + 1] - bounds[k] for k in range(nb)],
+                        "gcells_median": round(float(np.median(block_rates)), 4),
+                        "gcells_min": round(min(block_rates), 4), "gcells_max": round(max(block_rates), 4)})
 
 
 def run_cpu(size: int, seconds: float, q0: np.ndarray):
@@ -155,6 +169,15 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
         c, t = by_size.get(k, (0.0, 0.0))
         by_size[k] = (c + float(sp.cells) * sp.steps, t + svc[sp.idx])
     seq_by_size = {k: round(c / t / 1e9, 3) for k, (c, t) in sorted(by_size.items(), key=lambda kv: kv[1][0])}
+    # the reference's loop structure (dt to the host, one blocking step per RK2 step), one
+    # job at a time, on a seeded 10% subsample of the same stream
+    sub_host = sorted(int(i) for i in np.random.default_rng(seed + 3).choice(n_jobs, max(1, n_jobs // 10),
+                                                                             replace=False))
+    log(f"bench: {len(sub_host)} of the jobs one at a time through a host-driven step loop")
+    rh, _ = S.serve_sequential([specs[i] for i in sub_host], FP32, loop="host")
+    dev_sub = sum(svc[i] for i in sub_host)
+    log(f"bench: host loop {rh.throughput / 1e9:.2f} Gcell/s (device loop on the same jobs "
+        f"{rh.cell_steps / dev_sub / 1e9:.2f})")
     cap_jobs = n_jobs / rb.makespan
     rate = load * cap_jobs
     specs_rt = S.with_arrivals(specs, rate, seed=seed + 2)
@@ -191,6 +214,14 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
         "unit": "Gcell-updates/s",
         "one_at_a_time": round(rs.throughput / 1e9, 4),
         "one_at_a_time_by_grid": seq_by_size,
+        "one_at_a_time_host_loop": {
+            "jobs": len(sub_host),
+            "value": round(rh.throughput / 1e9, 4),
+            "device_loop_same_jobs": round(rh.cell_steps / dev_sub / 1e9, 4),
+            "speedup_of_batched": round(rb.throughput / rh.throughput, 3),
+            "note": "seeded 10% subsample, each job alone with the reference's loop structure: dt read back "
+                    "to the host and one blocking step per RK2 step (fv2d_compute_dt + fv2d_step)",
+        },
         "speedup_vs_one_at_a_time": round(rb.throughput / rs.throughput, 3),
         "makespan_s": {"batched": round(rb.makespan, 3), "one_at_a_time": round(rs.makespan, 3)},
         "gpu_busy_frac": round(rb.gpu_ms / 1e3 / rb.makespan, 3),
@@ -265,8 +296,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--blocks", type=int, default=5,
+                    help="the timed steps run as this many consecutive blocks, each event-timed (median reported)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -303,12 +336,12 @@ def main(argv=None):
     q0 = P.kh_ic(P.kh_params(N=a.size))
 
     log(f"bench: FP64 form, {a.warmup} warmup + {a.steps} timed steps")
-    r64 = run_gpu("fp64", a.size, a.steps, a.warmup, q0)
+    r64 = run_gpu("fp64", a.size, a.steps, a.warmup, q0, a.blocks)
     log(f"bench: FP64 {r64['gcells']:.3f} Gcell/s, {r64['ms_per_step']:.3f} ms/step")
     r32 = None
     if not a.no_fp32:
         log(f"bench: FP32 form, {a.warmup} warmup + {a.steps} timed steps")
-        r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0)
+        r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0, a.blocks)
         log(f"bench: FP32 {r32['gcells']:.3f} Gcell/s, {r32['ms_per_step']:.3f} ms/step")
     fp32_err = None
     if not a.no_fp32 and a.size == 4096:
@@ -347,6 +380,7 @@ def main(argv=None):
             "conduction_kappa": 1e-4,
             "boundaries": "periodic",
         },
+        "repetitions": r64["blocks"],
         "hbm_frac_step": round(BYTES_PER_CELL_STEP["fp64"] * cells / (r64["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
@@ -369,6 +403,7 @@ def main(argv=None):
             "value": round(r32["gcells"], 4),
             "ms_per_step": round(r32["ms_per_step"], 4),
             "speedup_vs_fp64": round(r32["gcells"] / r64["gcells"], 3),
+            "repetitions": r32["blocks"],
             "hbm_frac_step": round(BYTES_PER_CELL_STEP["fp32"] * cells / (r32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "roofline": {
                 "bound": "hbm",

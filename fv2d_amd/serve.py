@@ -209,11 +209,17 @@ def serve_batched(specs: list[JobSpec], precision: int = FP32, config: BatchConf
 
 
 def serve_sequential(specs: list[JobSpec], precision: int = FP32, realtime: bool = False,
-                     keep_prim: set | None = None, lib=None) -> tuple[ServeResult, dict]:
+                     keep_prim: set | None = None, lib=None, loop: str = "device") -> tuple[ServeResult, dict]:
     """Each job alone through the single-simulation path, in arrival order. With
-    realtime=False all jobs are taken as arrived at t = 0 (service times only)."""
+    realtime=False all jobs are taken as arrived at t = 0 (service times only).
+
+    loop="device": the time loop runs on the device (fv2d_advance). loop="host": the
+    reference's loop structure (main.cpp:62-84), dt read back to the host and one blocking
+    step per RK2 step (fv2d_compute_dt + fv2d_step)."""
+    if loop not in ("device", "host"):
+        raise ValueError(f"loop must be 'device' or 'host', not {loop!r}")
     keep_prim = keep_prim or set()
-    res = ServeResult("sequential")
+    res = ServeResult("sequential" if loop == "device" else "sequential-host-loop")
     prims = {}
     order = sorted(specs, key=lambda s: (s.arrival, s.idx))
     t0 = time.perf_counter()
@@ -225,7 +231,15 @@ def serve_sequential(specs: list[JobSpec], precision: int = FP32, realtime: bool
         adm = time.perf_counter() - t0
         with Solver(job_params(s, precision), lib=lib) as sv:
             sv.set_problem(s.kind, s.seed, s.amplitude)
-            t, _, _, n = sv.advance(max_steps=s.steps, record_dts=False)
+            if loop == "device":
+                t, _, _, n = sv.advance(max_steps=s.steps, record_dts=False)
+            else:
+                t, n = 0.0, 0
+                while n < s.steps:
+                    dt, _ = sv.compute_dt()
+                    sv.step(dt)
+                    t += dt
+                    n += 1
             sums = sv.sums()
             if s.idx in keep_prim:
                 prims[s.idx] = sv.get_prim()

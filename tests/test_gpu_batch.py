@@ -229,6 +229,21 @@ def test_serve_loop_small_stream(gpu_lib):
     assert sb == ss
 
 
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_host_step_loop_equals_device_loop(prec, gpu_lib):
+    """One job at a time with the host-driven loop (dt read back, one blocking step per RK2
+    step) gives the device loop's step count, time, sums and primitives, bitwise."""
+    specs = S.make_stream(6, rate=0, seed=11, steps=7, sizes=((128, 64), (96, 96)))
+    keep = {s.idx for s in specs}
+    rd, qd = S.serve_sequential(specs, prec, keep_prim=keep, lib=gpu_lib)
+    rh, qh = S.serve_sequential(specs, prec, keep_prim=keep, lib=gpu_lib, loop="host")
+    assert {r.idx: (r.nsteps, r.t, r.sums) for r in rd.records} == {r.idx: (r.nsteps, r.t, r.sums) for r in rh.records}
+    for i in keep:
+        assert np.array_equal(qd[i], qh[i]), f"job {i}"
+    with pytest.raises(ValueError):
+        S.serve_sequential(specs[:1], prec, lib=gpu_lib, loop="other")
+
+
 def _random_jobs(n, seed, prec):
     rng = np.random.default_rng(seed)
     jobs = []
