@@ -61,9 +61,19 @@ def test_grid_shapes_bitwise(Nx, Ny, emu_lib, oracle_lib):
         compare(p, q0, emu_lib, oracle_lib, steps=2)
 
 
+@pytest.mark.parametrize("Nx,Ny,seg", [(124, 66, 8), (125, 130, 12), (130, 7, 4), (5, 301, 0), (62, 4, 0)])
+def test_segment_heights_bitwise(Nx, Ny, seg, emu_lib, oracle_lib):
+    """Segments of 4, 8 and 12 rows (a last segment shorter than a chunk, several segments
+    per strip), exactly one strip, a tall grid with the default segment height."""
+    for n, c in enumerate([(1, 1, PLM, 1, 1, 1, 1, 1, 1), (2, 0, PLM, 0, 0, 0, 1, 0, 0)]):
+        p, q0 = make(n, *c, Nx=Nx, Ny=Ny)
+        compare(p.copy(seg_rows=seg), q0, emu_lib, oracle_lib, steps=2)
+
+
 def test_sanitized_bounds(oracle_lib):
-    """ASan/UBSan build: every 9th combination, 2 steps; a bad index aborts the process, so
-    this test runs the sanitized library in a child interpreter."""
+    """ASan/UBSan build: every 9th combination and three strip / segment edge shapes, 2 steps;
+    a bad index aborts the process, so this test runs the sanitized library in a child
+    interpreter."""
     import subprocess
     import sys
     EMU.build()
@@ -76,6 +86,9 @@ def test_sanitized_bounds(oracle_lib):
         "    if n %% 9: continue\n"
         "    p, q0 = T.make(n, *c)\n"
         "    T.compare(p, q0, lib, olib, steps=2)\n"
+        "for Nx, Ny, seg in ((124, 66, 8), (130, 7, 4), (2, 2, 0)):\n"
+        "    p, q0 = T.make(0, 1, 1, 2, 1, 1, 1, 1, 1, 1, Nx=Nx, Ny=Ny)\n"
+        "    T.compare(p.copy(seg_rows=seg), q0, lib, olib, steps=2)\n"
         "T.fp32_close(lib, olib, N=24, steps=4)\n"
         "print('ok')\n" % ([os.path.dirname(__file__), os.path.dirname(os.path.dirname(__file__)),
                            os.path.join(os.path.dirname(__file__), "emu")],))

@@ -83,6 +83,28 @@ def test_switch_combos_random_state(bc, recon, riem, ts, oracle_lib, gpu_lib):
     assert run_pair(p, q0, 4, oracle_lib, gpu_lib)
 
 
+SHAPE_SWITCHES = [  # (bcx, bcy, recon, riem, ts, tc, visc, grav, wb)
+    (BC_REFLECTING, BC_REFLECTING, PLM, HLLC, 1, 1, 1, GRAV_CONSTANT, 1),
+    (BC_PERIODIC, BC_ABSORBING, PLM, HLL, 0, 0, 1, GRAV_NONE, 0),
+    (BC_ABSORBING, BC_PERIODIC, PCM, FSLP, 1, 1, 0, GRAV_ANALYTICAL, 1),
+]
+
+
+@pytest.mark.parametrize("Nx,Ny,seg", [(2, 2, 0), (3, 5, 0), (62, 4, 0), (63, 9, 0), (124, 66, 8), (125, 130, 12),
+                                       (5, 301, 0), (130, 7, 4), (300, 3, 0)])
+def test_grid_shapes_bitwise(Nx, Ny, seg, oracle_lib, gpu_lib):
+    """Grids at the edges of the strip and segment decomposition (62 output columns per
+    strip, segments of `seg` rows, 4-row chunks): the smallest grid, one column over a strip,
+    exactly one and two strips, a last segment of a few rows, tall and flat grids."""
+    for n, (bcx, bcy, recon, riem, ts, tc, vi, grav, wb) in enumerate(SHAPE_SWITCHES):
+        p = Params.defaults(Nx=Nx, Ny=Ny, xmin=0.1, xmax=1.3, ymin=-0.2, ymax=0.7, boundary_x=bcx, boundary_y=bcy,
+                            reconstruction=recon, riemann_solver=riem, time_stepping=ts, tc_active=tc, kappa=3e-3,
+                            visc_active=vi, mu=2e-3, gravity_mode=grav, gx=0.3, gy=-0.7, hot_bubble_g0=0.5,
+                            well_balanced_flux_at_y_bc=wb, gamma0=1.4, CFL=0.4, seg_rows=seg)
+        q0 = P.random_state(p, seed=1000 * n + Nx + Ny)
+        assert run_pair(p, q0, 3, oracle_lib, gpu_lib), f"{Nx}x{Ny} seg {seg} switches {n}: not bitwise"
+
+
 @pytest.mark.parametrize("grav,wb,bctc", [(GRAV_NONE, 0, 0), (GRAV_CONSTANT, 1, 0), (GRAV_ANALYTICAL, 1, 1),
                                           (GRAV_CONSTANT, 0, 2)])
 def test_gravity_wb_wall_conduction(grav, wb, bctc, oracle_lib, gpu_lib):
