@@ -56,11 +56,33 @@ FV2D_HD double markstein_div(double a, double d, double rd, int fast) {
   return a / d;
 }
 
+// IEEE square root of a double. On the device (gfx950) the compiler's sequence scales inputs
+// below 2^-767 by 2^256 (and the result back by 2^-128), runs v_rsq_f64 and two Newton
+// steps (10 instructions), and returns +-0 and +inf unchanged. For x in [2^-767, DBL_MAX]
+// the scale factors are 2^0 and the special-value test fails, so the core alone gives the
+// same bits; other inputs take the full sequence.
+FV2D_HD double sqrt_ieee(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+  }
+#endif
+  return sqrt(x);
+}
+
 template <>
 struct Ops<double> {
   static FV2D_HD double div(double a, double b) { return a / b; }
   static FV2D_HD double div_c(double a, double d, double rd, int fast) { return markstein_div(a, d, rd, fast); }
-  static FV2D_HD double sqrt_(double a) { return sqrt(a); }
+  static FV2D_HD double sqrt_(double a) { return sqrt_ieee(a); }
   static FV2D_HD double min_(double a, double b) { return fmin(a, b); }
   static FV2D_HD double max_(double a, double b) { return fmax(a, b); }
 };
@@ -89,6 +111,52 @@ struct Ops<float> {
   static FV2D_HD float min_(float a, float b) { return fminf(a, b); }
   static FV2D_HD float max_(float a, float b) { return fmaxf(a, b); }
 };
+
+// Quotients a_k / b of several numerators by one divisor (the two velocities of
+// consToPrim and T = p / rho, uS and pS of HLLC, its star density and energy, the four HLL
+// star fluxes). The compiler's FP64 division on the device is: ds = v_div_scale(b, for a),
+// r = v_rcp_f64(ds), two Newton steps on r (4 FMAs), then ns = v_div_scale(a), m = ns * r,
+// one FMA correction, v_div_fmas and v_div_fixup. The first half sees a only through the
+// scaling of b, which is the identity unless a, b or a / b lie near the ends of the exponent
+// range. DivBy runs it once for the first numerator; for each further numerator it
+// recomputes the scaled divisor (one instruction) and reuses the refined reciprocal when the
+// two are bitwise equal, so every quotient has the bits of a / b (and is a / b when they
+// differ). On the host and in the FP32 form (where division is a * rcp(b), the reciprocal
+// shared by the compiler) it is plain division.
+template <typename R>
+struct DivBy {
+  R b, a0;
+  FV2D_HD DivBy(R b_, R a0_) : b(b_), a0(a0_) {}
+  FV2D_HD R first() const { return Ops<R>::div(a0, b); }
+  FV2D_HD R operator()(R a) const { return Ops<R>::div(a, b); }
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+template <>
+struct DivBy<double> {
+  double b, a0, ds, rf;
+  __device__ DivBy(double b_, double a0_) : b(b_), a0(a0_) {
+    bool scale;
+    ds = __builtin_amdgcn_div_scale(a0, b, false, &scale);
+    const double r = __builtin_amdgcn_rcp(ds);
+    const double r1 = fma(r, fma(-ds, r, 1.0), r);
+    rf = fma(r1, fma(-ds, r1, 1.0), r1);
+  }
+  // the second half of the division, for a numerator whose scaled divisor is ds
+  __device__ double quot(double a) const {
+    bool scale;
+    const double ns = __builtin_amdgcn_div_scale(a, b, true, &scale);
+    const double m = ns * rf;
+    return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(fma(-ds, m, ns), rf, m, scale), b, a);
+  }
+  __device__ double first() const { return quot(a0); }
+  __device__ double operator()(double a) const {
+    bool scale;
+    if (__builtin_amdgcn_div_scale(a, b, false, &scale) == ds) return quot(a);
+    return a / b;
+  }
+};
+#endif
 
 // Physics constants of one simulation, precomputed on the host with the reference's own
 // expressions (e.g. gm1 = gamma0 - 1.0, entho = 1.0 / (gamma0 - 1.0)), so the kernels
@@ -131,15 +199,18 @@ FV2D_HD St<R> prim_to_cons(const St<R> &q, R gm1) {
 
 // consToPrim (States.h:32-43)
 template <typename R>
-FV2D_HD St<R> cons_to_prim(const St<R> &u, R gm1) {
-  typedef Ops<R> O;
+FV2D_HD St<R> cons_to_prim(const St<R> &u, R gm1, const DivBy<R> &byr) {
   St<R> r;
   r.v[IR] = u.v[IR];
-  r.v[IU] = O::div(u.v[IU], u.v[IR]);
-  r.v[IV] = O::div(u.v[IV], u.v[IR]);
+  r.v[IU] = byr.first();
+  r.v[IV] = byr(u.v[IV]);
   R Ek = R(0.5) * r.v[IR] * (r.v[IU] * r.v[IU] + r.v[IV] * r.v[IV]);
   r.v[IP] = (u.v[IE] - Ek) * gm1;
   return r;
+}
+template <typename R>
+FV2D_HD St<R> cons_to_prim(const St<R> &u, R gm1) {
+  return cons_to_prim(u, gm1, DivBy<R>(u.v[IR], u.v[IU]));
 }
 
 // Negative density / pressure clamp of checkNegatives (SimInfo.h:617-626), on one cell.
@@ -225,8 +296,13 @@ FV2D_HD void hll(const St<R> &qL, const St<R> &qR, R gamma0, R gm1, St<R> &flux,
     pout = R(0.5) * (qL.v[IP] + qR.v[IP]);
     const R SLSR = SL * SR;
     const R den = SR - SL;
+    R num[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) flux.v[k] = O::div(FL.v[k] * SR - FR.v[k] * SL + (uR.v[k] - uL.v[k]) * SLSR, den);
+    for (int k = 0; k < 4; ++k) num[k] = FL.v[k] * SR - FR.v[k] * SL + (uR.v[k] - uL.v[k]) * SLSR;
+    const DivBy<R> byd(den, num[0]);
+    flux.v[0] = byd.first();
+#pragma unroll
+    for (int k = 1; k < 4; ++k) flux.v[k] = byd(num[k]);
   }
 }
 
@@ -254,8 +330,9 @@ FV2D_HD void hllc(const St<R> &qL, const St<R> &qR, R gamma0, R entho, St<R> &fl
   const R rcR = rR * (SR - uR);
 
   const R den = rcR + rcL;
-  const R uS = O::div(rcR * uR + rcL * uL + (pL - pR), den);
-  const R pS = O::div(rcR * pL + rcL * pR + rcL * rcR * (uL - uR), den);
+  const DivBy<R> byd(den, rcR * uR + rcL * uL + (pL - pR));
+  const R uS = byd.first();
+  const R pS = byd(rcR * pL + rcL * pR + rcL * rcR * (uL - uR));
 
   // Branches SL > 0, uS > 0, SR > 0, else (RiemannSolvers.h:93-122)
   R sr, su, sv, sp, E;
@@ -275,8 +352,9 @@ FV2D_HD void hllc(const St<R> &qL, const St<R> &qR, R gamma0, R entho, St<R> &fl
     const R EX = left ? EL : ER;
     const R SX = left ? SL : SR;
     const R dS = SX - uS;
-    sr = O::div(rX * (SX - uX), dS);
-    E = O::div((SX - uX) * EX - pX * uX + pS * uS, dS);
+    const DivBy<R> bys(dS, rX * (SX - uX));
+    sr = bys.first();
+    E = bys((SX - uX) * EX - pX * uX + pS * uS);
     su = uS;
     sv = left ? vL : vR;
     sp = pS;
@@ -284,8 +362,9 @@ FV2D_HD void hllc(const St<R> &qL, const St<R> &qR, R gamma0, R entho, St<R> &fl
   } else if (uS > R(0)) {
     // SR <= 0 < uS cannot happen for ordered speeds, but the reference tests uS first
     const R dS = SL - uS;
-    sr = O::div(rL * (SL - uL), dS);
-    E = O::div((SL - uL) * EL - pL * uL + pS * uS, dS);
+    const DivBy<R> bys(dS, rL * (SL - uL));
+    sr = bys.first();
+    E = bys((SL - uL) * EL - pL * uL + pS * uS);
     su = uS;
     sv = vL;
     sp = pS;
@@ -373,10 +452,9 @@ FV2D_HD int map_ghost(int g, int N, int bc, bool &flip) {
   return s;
 }
 
-// The hyperbolic inverse time step of one cell (ComputeDt.h:31-34).
+// The hyperbolic inverse time step of one cell (ComputeDt.h:31-34), given its sound speed.
 template <typename R>
-FV2D_HD double inv_dt_hyp(const St<R> &q, const KParams &kp) {
-  const R cs = speed_of_sound(q, R(kp.gamma0));
+FV2D_HD double inv_dt_hyp_cs(const St<R> &q, R cs, const KParams &kp) {
   if (sizeof(R) == 8) {
     return markstein_div((double)(cs + fabs(q.v[IU])), kp.dx, kp.inv_dx, kp.fastdiv) +
            markstein_div((double)(cs + fabs(q.v[IV])), kp.dy, kp.inv_dy, kp.fastdiv);
@@ -384,6 +462,11 @@ FV2D_HD double inv_dt_hyp(const St<R> &q, const KParams &kp) {
     // FP32 form: FP64 for the reduction, per-cell value from the FP32 state
     return ((double)cs + fabs((double)q.v[IU])) * kp.inv_dx + ((double)cs + fabs((double)q.v[IV])) * kp.inv_dy;
   }
+}
+
+template <typename R>
+FV2D_HD double inv_dt_hyp(const St<R> &q, const KParams &kp) {
+  return inv_dt_hyp_cs(q, speed_of_sound(q, R(kp.gamma0)), kp);
 }
 
 }  // namespace fv2d

@@ -189,35 +189,45 @@ struct StripStage {
     for (int v = 0; v < 4; ++v) raw[v] = base[v * kp.plane + k];
   }
 
-  // Primitives from storage values: Q0 as given, else consToPrim (+ reset).
+  // Primitives from storage values: Q0 as given, else consToPrim (+ reset); with conduction
+  // also T = p / rho (ThermalConduction.h:56-60), which shares consToPrim's divisor rho
+  // unless the reset changed it.
   static FV2D_HD void raw_to_q(const S raw[4], bool fx, bool fy, const KParams &kp, const StageIO<S> &io,
-                               St<R> &q) {
+                               St<R> &q, R &t) {
     if (io.qsrc == QSRC_Q0) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) q.v[v] = R(raw[v]);
+      if (TC) t = Ops<R>::div(q.v[IP], q.v[IR]);
     } else {
       St<R> u;
 #pragma unroll
       for (int v = 0; v < 4; ++v) u.v[v] = R(raw[v]);
-      q = cons_to_prim(u, R(kp.gm1));
+      const DivBy<R> byr(u.v[IR], u.v[IU]);
+      q = cons_to_prim(u, R(kp.gm1), byr);
       if (io.qsrc == QSRC_U_RESET) reset_negative(q, R(kp.eps_reset));
+      if (TC) {
+        if (q.v[IR] == u.v[IR])
+          t = byr(q.v[IP]);
+        else
+          t = Ops<R>::div(q.v[IP], q.v[IR]);
+      }
     }
     if (fx) q.v[IU] = -q.v[IU];
     if (fy) q.v[IV] = -q.v[IV];
   }
 
-  // Primitive state of domain-relative cell (gi, gj), ghosts included.
-  static FV2D_HD void fetch_q(int gi, int gj, const KParams &kp, const StageIO<S> &io, St<R> &q) {
+  // Primitive state (and T) of domain-relative cell (gi, gj), ghosts included.
+  static FV2D_HD void fetch_q(int gi, int gj, const KParams &kp, const StageIO<S> &io, St<R> &q, R &t) {
     S raw[4];
     bool fx, fy;
     fetch_raw(gi, gj, kp, io, raw, fx, fy);
-    raw_to_q(raw, fx, fy, kp, io, q);
+    raw_to_q(raw, fx, fy, kp, io, q, t);
   }
 
-  static FV2D_HD void put(Smem &sm, int slot, int col, const St<R> &q) {
+  static FV2D_HD void put(Smem &sm, int slot, int col, const St<R> &q, R t) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) sm.q[v][slot][col] = q.v[v];
-    if (TC) sm.t[TC ? slot : 0][TC ? col : 0] = Ops<R>::div(q.v[IP], q.v[IR]);  // T = p / rho
+    if (TC) sm.t[TC ? slot : 0][TC ? col : 0] = t;
   }
 
   static FV2D_HD R Q(const Smem &sm, int v, int j, int col, int jb) { return sm.q[v][qslot(j, jb)][col]; }
@@ -230,8 +240,9 @@ struct StripStage {
   static FV2D_HD void load_row(int lane, int w, int j, int i0, int jb, Smem &sm, const KParams &kp,
                                const StageIO<S> &io) {
     St<R> q;
-    fetch_q(i0 - 2 + lane, j, kp, io, q);
-    put(sm, qslot(j, jb), lane, q);
+    R t = R(0);
+    fetch_q(i0 - 2 + lane, j, kp, io, q, t);
+    put(sm, qslot(j, jb), lane, q, t);
     (void)w;
   }
   static FV2D_HD void load_extra(int lane, int j0, int nrows, int i0, int jb, Smem &sm, const KParams &kp,
@@ -239,8 +250,9 @@ struct StripStage {
     if (lane < 2 * nrows) {
       const int j = j0 + (lane >> 1), col = STRIP_W + (lane & 1);
       St<R> q;
-      fetch_q(i0 - 2 + col, j, kp, io, q);
-      put(sm, qslot(j, jb), col, q);
+      R t = R(0);
+      fetch_q(i0 - 2 + col, j, kp, io, q, t);
+      put(sm, qslot(j, jb), col, q, t);
     }
   }
 
@@ -368,13 +380,14 @@ struct StripStage {
   static FV2D_HD void chunk_commit(int lane, int w, int xw, int i0, int jb, int jc, Smem &sm, const KParams &kp,
                                    const StageIO<S> &io, const RowPrefetch<S> &p, LaneRegs<R> &r) {
     St<R> q;
-    raw_to_q(p.row, (p.flags & 1) != 0, (p.flags & 2) != 0, kp, io, q);
-    put(sm, qslot(jc + 2 + w, jb), lane, q);
+    R t = R(0);
+    raw_to_q(p.row, (p.flags & 1) != 0, (p.flags & 2) != 0, kp, io, q, t);
+    put(sm, qslot(jc + 2 + w, jb), lane, q, t);
     if (w == xw) {
       if (kPFX) {
         if (lane < 8) {
-          raw_to_q(p.extra, (p.flags & 4) != 0, (p.flags & 8) != 0, kp, io, q);
-          put(sm, qslot(jc + 2 + (lane >> 1), jb), STRIP_W + (lane & 1), q);
+          raw_to_q(p.extra, (p.flags & 4) != 0, (p.flags & 8) != 0, kp, io, q, t);
+          put(sm, qslot(jc + 2 + (lane >> 1), jb), STRIP_W + (lane & 1), q, t);
         }
       } else {
         load_extra(lane, jc + 2, 4, i0, jb, sm, kp, io);
@@ -604,7 +617,8 @@ struct StripStage {
 
     // epilogue: consToPrim + checkNegatives (SimInfo.h:576-646) and the next dt's cell term
     if (io.epilogue) {
-      St<R> q = cons_to_prim(un, R(kp.gm1));
+      const DivBy<R> byr(un.v[IR], un.v[IU]);
+      St<R> q = cons_to_prim(un, R(kp.gm1), byr);
       const R eps = R(kp.eps_reset);
       if (q.v[IR] < R(0)) {
         q.v[IR] = eps;
@@ -616,7 +630,13 @@ struct StripStage {
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) r.nan_cnt += (q.v[v] != q.v[v]) ? 1u : 0u;
-      const double inv = inv_dt_hyp(q, kp);
+      // speedOfSound (States.h:45-46): (p * gamma) / rho, by the same divisor unless reset
+      R c2;
+      if (q.v[IR] == un.v[IR])
+        c2 = byr(q.v[IP] * R(kp.gamma0));
+      else
+        c2 = Ops<R>::div(q.v[IP] * R(kp.gamma0), q.v[IR]);
+      const double inv = inv_dt_hyp_cs(q, Ops<R>::sqrt_(c2), kp);
       if (inv > r.inv) r.inv = inv;  // NaN never wins: the reference's fmax-based Max ignores it
     }
   }
