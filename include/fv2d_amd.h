@@ -49,8 +49,9 @@ enum fv2d_gravity_mode { FV2D_GRAV_NONE = 0, FV2D_GRAV_CONSTANT = 1, FV2D_GRAV_A
 enum fv2d_tc_mode { FV2D_TCM_CONSTANT = 0, FV2D_TCM_B02 = 1 };
 enum fv2d_bctc_mode { FV2D_BCTC_NONE = 0, FV2D_BCTC_FIXED_TEMPERATURE = 1, FV2D_BCTC_FIXED_GRADIENT = 2 };
 /* Arithmetic form. FP64 is the parity form. FP32 runs reconstruction, Riemann fluxes and the
- * parabolic/gravity stencils in FP32, stores the state in FP32 and keeps the conserved-state
- * accumulation U += dt*dF, the conservation sums and the dt reduction in FP64. */
+ * parabolic/gravity stencils in FP32 and stores the state in FP32; the step's increment dt*dF
+ * is formed in FP32 and added to U in FP64 (with the RK2 average), and the conservation sums
+ * and the dt reduction are FP64. */
 enum fv2d_precision { FV2D_FP64 = 0, FV2D_FP32 = 1 };
 
 /*
