@@ -1,6 +1,6 @@
 // Test infrastructure: runs the FP64 quotient and square-root helpers of the stage kernel
-// (DivBy, sqrt_ieee in fv2d_amd/csrc/fv2d_math.h) on the GPU next to the compiler's own
-// a / b and sqrt, for a host-side bitwise comparison. Built with the flags of the FP64 units.
+// (DivBy, sqrt_ieee, markstein_div in fv2d_amd/csrc/fv2d_math.h) on the GPU next to the
+// compiler's own a / b and sqrt, for a host-side bitwise comparison. Built with the flags of the FP64 units.
 #include <hip/hip_runtime.h>
 
 #include "../../fv2d_amd/csrc/fv2d_math.h"
@@ -20,6 +20,32 @@ __global__ void probe_kernel(const double *a0, const double *a1, const double *b
   out[6 * i + 3] = x1 / d;
   out[6 * i + 4] = sqrt_ieee(x0);
   out[6 * i + 5] = sqrt(x0);
+}
+
+// Per i: out[2i] = markstein_div(a[i], d, rd, fast), out[2i + 1] = a[i] / d.
+__global__ void markstein_kernel(const double *a, int n, double d, double rd, int fast, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[2 * i] = markstein_div(a[i], d, rd, fast);
+  out[2 * i + 1] = a[i] / d;
+}
+
+extern "C" int markstein_probe(const double *a, int n, double d, double rd, int fast, double *out) {
+  if (n <= 0) return -1;
+  double *m = nullptr;
+  const size_t in_bytes = sizeof(double) * (size_t)n;
+  if (hipMalloc((void **)&m, 3 * in_bytes) != hipSuccess) return -2;
+  int rc = 0;
+  if (hipMemcpy(m, a, in_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = -3;
+  } else {
+    hipLaunchKernelGGL(markstein_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, m, n, d, rd, fast, m + n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, m + n, 2 * in_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = -4;
+  }
+  (void)hipFree(m);
+  return rc;
 }
 
 extern "C" int divsqrt_probe(const double *a0, const double *a1, const double *b, int n, double *out) {

@@ -1,6 +1,7 @@
 """The FP64 quotient and square-root helpers of the stage kernel on the GPU
 (fv2d_amd/csrc/fv2d_math.h: DivBy, which reuses one refined reciprocal for several
-numerators, and sqrt_ieee, which skips the scaling steps for in-range inputs) against the
+numerators, sqrt_ieee, which skips the scaling steps for in-range inputs, and the
+constant-divisor markstein_div, whose waves share the fast path) against the
 compiler's own a / b and sqrt in the same kernel, bit for bit (NaNs as NaNs): operands
 log-uniform over the whole exponent range with both signs, subnormals, zeros, infinities and
 NaNs, numerator pairs of very different magnitude (the reuse check fails and a / b is taken),
@@ -24,6 +25,9 @@ def probe():
     lib = C.CDLL(LIB)
     lib.divsqrt_probe.argtypes = [C.POINTER(C.c_double)] * 3 + [C.c_int, C.POINTER(C.c_double)]
     lib.divsqrt_probe.restype = C.c_int
+    lib.markstein_probe.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.c_double, C.c_int,
+                                    C.POINTER(C.c_double)]
+    lib.markstein_probe.restype = C.c_int
     return lib
 
 
@@ -93,3 +97,29 @@ def test_hydro_magnitudes(probe):
     check(run(probe, mom, prs, rho))
     check(run(probe, prs * 1.4, mom * 0.5, rho))
     check(run(probe, rng.normal(0.0, 1e-3, n), rng.normal(0.0, 1e-9, n), rng.normal(0.0, 1.0, n)))
+
+
+def test_markstein_on_device(probe):
+    """The constant-divisor division (markstein_div) on the device, where lanes of one wave
+    share the fast path or fall back together: cell sizes and random divisors, numerators
+    over the whole range with zeros of both signs and special values mixed into every wave."""
+    rng = np.random.default_rng(10)
+    n = 1 << 18
+    for d in (1.0 / 4096, 1.0 / 2048, 0.02 / 512, 1.5 / 3072, 0.5 / 1024) + tuple(
+            float(np.ldexp(rng.uniform(1, 2), int(rng.integers(-60, 61)))) for _ in range(12)):
+        a = np.concatenate([loguniform(rng, n), rng.uniform(-1e3, 1e3, n), np.zeros(64), -np.zeros(64),
+                            np.tile(SPECIALS, 8)])
+        rng.shuffle(a)
+        a = np.ascontiguousarray(a)
+        out = np.zeros(2 * a.size)
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        fast = 1 if 2.0 ** -60 <= d <= 2.0 ** 60 else 0
+        assert probe.markstein_probe(dp(a), a.size, d, 1.0 / d, fast, dp(out)) == 0
+        out = out.reshape(-1, 2)
+        bad = ~same_bits(out[:, 0], out[:, 1])
+        assert not bad.any(), f"d={d!r}: {np.count_nonzero(bad)} mismatches, e.g. a={a[np.flatnonzero(bad)[:3]]}"
+        # whole waves of zeros and of ordinary values take the fast path
+        z = np.ascontiguousarray(np.concatenate([np.zeros(256), -np.zeros(256), rng.uniform(1, 2, 256)]))
+        out = np.zeros(2 * z.size)
+        assert probe.markstein_probe(dp(z), z.size, d, 1.0 / d, fast, dp(out)) == 0
+        assert same_bits(out[0::2], out[1::2]).all()
