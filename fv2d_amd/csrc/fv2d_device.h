@@ -93,8 +93,11 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
     K::pass1_slopes(lane, w, jb, jc, sm, r);
 #pragma unroll
     for (int v = 0; v < 4; ++v) r.sxl[v] = lane_shr1(r.sx[v]);
-    K::pass1_flux(lane, w, jb, jc, sm, kp, r);
     __syncthreads();
+    // the x and the y interface: two independent Riemann solves in one phase, so the
+    // compiler can interleave their dependency chains (the x solve reads only registers and
+    // the primitive ring, which the barrier does not concern)
+    K::pass1_flux(lane, w, jb, jc, sm, kp, r);
     K::pass2_flux(lane, w, jb, jc, sm, kp, r);
     __syncthreads();
     K::update_faces(lane, w, jb, jc, sm, kp, r);
