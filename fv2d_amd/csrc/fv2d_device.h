@@ -94,12 +94,11 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
 #pragma unroll
     for (int v = 0; v < 4; ++v) r.sxl[v] = lane_shr1(r.sx[v]);
     __syncthreads();
-    // the x and the y interface: two independent Riemann solves in one phase, so the
-    // compiler can interleave their dependency chains (the x solve reads only registers and
-    // the primitive ring, which the barrier does not concern)
+    // the x and the y interface and the x-face conduction / viscous terms in one phase:
+    // independent dependency chains the compiler can interleave (the x-face work reads only
+    // registers and the primitive ring, which is stable since the load barrier)
     K::pass1_flux(lane, w, jb, jc, sm, kp, r);
     K::pass2_flux(lane, w, jb, jc, sm, kp, r);
-    __syncthreads();
     K::update_faces(lane, w, jb, jc, sm, kp, r);
 #pragma unroll
     for (int v = 0; v < 4; ++v) r.fxl[v] = lane_shr1(r.fx[v]);
@@ -108,6 +107,7 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
 #pragma unroll
       for (int v = 0; v < 3; ++v) r.vxl[v] = lane_shr1(r.vx[v]);
     }
+    __syncthreads();
     K::update_cell(lane, w, i0, jb, jc, je, sm, kp, io, dt, dt_dx, dt_dy, r);
     __syncthreads();
     xw = (xw + 1) & (STRIP_ROWS - 1);
