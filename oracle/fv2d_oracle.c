@@ -1,5 +1,5 @@
 /*
- * fv2d_oracle.c -- TEST INFRASTRUCTURE ONLY (parity checker and CPU baseline).
+ * fv2d_oracle.c -- TEST INFRASTRUCTURE ONLY (parity reference and CPU baseline).
  *
  * A plain-C, FP64 restatement of the reference hot path of mdelorme/fv2d. Each function
  * cites the reference file:line it follows and keeps the reference's operation order, so

@@ -2,7 +2,7 @@
  * fv2d_oracle.h -- TEST INFRASTRUCTURE ONLY.
  *
  * CPU FP64 restatement of the reference (mdelorme/fv2d) hot path, used as the parity
- * checker for the HIP path and as the timed CPU baseline. Only tests/, the smoke check and
+ * reference of the HIP path and as the timed CPU baseline. Only tests/, the smoke check and
  * the CPU-baseline leg of bench.py may load it. The product never links or calls it.
  *
  * The oracle keeps the reference's data model: full arrays with Ng ghost layers

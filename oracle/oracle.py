@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle (liboracle_fv2d.so).
 
 Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` may use
-this module, and only as the checker / the timed CPU baseline. The product path never
+this module, and only as the parity reference / the timed CPU baseline. The product path never
 imports it.
 """
 from __future__ import annotations
