@@ -34,9 +34,7 @@ COMMON = [
 # (source, extra flags)
 UNITS = [
     ("stage_fp64.hip", ["-ffp-contract=off"]),
-    # 5 waves per SIMD: the FP32 kernels fit 96 VGPRs (the conduction + viscosity batch
-    # classes with a few spills), which measured faster than 4 waves with no spills
-    ("stage_fp32.hip", ["-ffp-contract=fast", "-DFV2D_WAVES_PER_EU=5"]),
+    ("stage_fp32.hip", ["-ffp-contract=fast"]),
     ("fv2d_capi.hip", ["-ffp-contract=off"]),
     ("fv2d_batch.hip", ["-ffp-contract=off"]),
 ]

@@ -49,10 +49,14 @@ __device__ inline void commit_epilogue(unsigned nr, unsigned np, unsigned nn, do
   }
 }
 
-// Ask the compiler for 4 waves per SIMD (at most 128 VGPRs): with a 40 KB LDS image, 4 workgroups
-// of 4 waves fill a CU; the FP64 KH variant would otherwise take 133 VGPRs and 3 waves.
-#ifndef FV2D_WAVES_PER_EU
-#define FV2D_WAVES_PER_EU 4
+// Waves per SIMD asked of the compiler: K::kWPE / K::kWPEBatch per kernel class (fv2d_stage.h), or one value
+// for every class when FV2D_WAVES_PER_EU is defined (build variants for measurements).
+#ifdef FV2D_WAVES_PER_EU
+#define FV2D_WPE(K) FV2D_WAVES_PER_EU
+#define FV2D_WPE_BATCH(K) FV2D_WAVES_PER_EU
+#else
+#define FV2D_WPE(K) K::kWPE
+#define FV2D_WPE_BATCH(K) K::kWPEBatch
 #endif
 
 // One workgroup's share of a stage: strip/segment `blk` of the simulation described by kp,
@@ -116,7 +120,7 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
 }
 
 template <class K, typename R, typename S>
-__global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D_WAVES_PER_EU)))
+__global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D_WPE(K))))
 strip_kernel(const KParams kp, const StageIO<S> io, const StripGeom g) {
   __shared__ typename K::Smem sm;
   const StepCtl *ctl = io.ctl;
@@ -130,7 +134,7 @@ strip_kernel(const KParams kp, const StageIO<S> io, const StripGeom g) {
 // stage 1, with U / Us alternating by step parity. A job's arithmetic does not depend on
 // the other jobs, so it is bitwise the same alone or in the mix.
 template <class K, typename R, typename S>
-__global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D_WAVES_PER_EU)))
+__global__ void __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(FV2D_WPE_BATCH(K))))
 batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restrict__ items, const int *range,
                    int *counter, int stage) {
   __shared__ typename K::Smem sm;

@@ -151,6 +151,13 @@ struct StripStage {
   static constexpr int NY = 6;            // y-slope / y-flux ring rows
   static constexpr int NV = 5;            // viscous y-face ring rows
   static constexpr bool kTC = TC, kVISC = VISC;
+  // Waves per SIMD asked of the compiler (its VGPR budget). FP64: 4 (at most 128 VGPRs; the
+  // 39 KB LDS image allows 4 workgroups of 4 waves per CU). FP32: 5 (96 VGPRs); the
+  // single-grid kernel of the classes without conduction, viscosity and the gravity / wall
+  // terms fits 6 (80 VGPRs), where the batch kernel (work-list state) and the other classes
+  // would spill. Each measured faster than one wave fewer.
+  static constexpr int kWPE = sizeof(R) == 8 ? 4 : ((TC || VISC || XG) ? 5 : 6);
+  static constexpr int kWPEBatch = sizeof(R) == 8 ? 4 : 5;
 #ifndef FV2D_PREFETCH_FP64
 #define FV2D_PREFETCH_FP64 0
 #endif
