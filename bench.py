@@ -11,7 +11,8 @@ synthetic seeded initial state, all state resident in HBM. One step = one full R
 (Gcell-updates/s = Nx*Ny*K / time of K steps, HIP events on the solver stream, device
 synchronised on both sides); the FP32 form is timed the same way and reported beside it.
 The K steps run as --blocks consecutive device loops with an event between them:
-`repetitions` gives the median, minimum and maximum rate of the blocks.
+`repetitions` gives the median, minimum and maximum rate of the blocks. `pcie_inclusive`
+adds the upload of the host primitives before and their read-back after (host clock).
 
 roofline: the dominant kernel is the fused stage kernel (two launches per step). Its
 algorithmic HBM bytes per cell are 160 B per FP64 RK2 step (stage 1 reads U^n and writes
@@ -77,7 +78,11 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, block
 
     p = P.kh_params(N=size, precision=FP64 if prec == "fp64" else FP32)
     s = Solver(p)
-    s.set_prim(q0)
+    hiprt.synchronize()
+    t_up = time.perf_counter()
+    s.set_prim(q0)  # host primitives over PCIe, then U = primToCons on the device
+    hiprt.synchronize()
+    t_up = time.perf_counter() - t_up
     if warmup:
         s.advance(max_steps=warmup, record_dts=False)
     stream = s.lib.fv2d_get_stream(s.h)
@@ -110,11 +115,15 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, block
     mean_launch_s = st_ms.value * 1e-3 / max(st_n.value, 1)
     bpl = BYTES_PER_CELL_STEP[prec] / STAGE_LAUNCHES_PER_STEP
     achieved = bpl * cells / mean_launch_s / 1e9
-    q = s.get_prim()
+    t_down = time.perf_counter()
+    q = s.get_prim()  # primitives back to the host
+    t_down = time.perf_counter() - t_down
     sums = s.sums()
     s.close()
     return dict(gcells=gcells, ms_per_step=ms / steps, mean_launch_ms=mean_launch_s * 1e3, launches=int(st_n.value),
                 achieved=achieved, bytes_per_launch=bpl * cells, counts=counts, q=q, sums=sums,
+                pcie={"upload_ms": round(t_up * 1e3, 2), "download_ms": round(t_down * 1e3, 2),
+                      "gcells": round(cells * steps / (ms * 1e-3 + t_up + t_down) / 1e9, 4)},
                 blocks={"n": nb, "steps": [bounds[k + 1] - bounds[k] for k in range(nb)],
                         "gcells_median": round(float(np.median(block_rates)), 4),
                         "gcells_min": round(min(block_rates), 4), "gcells_max": round(max(block_rates), 4)})
@@ -386,6 +395,8 @@ def main(argv=None):
             "boundaries": "periodic",
         },
         "repetitions": r64["blocks"],
+        "pcie_inclusive": dict(r64["pcie"], note="the same K steps with the host primitives uploaded before "
+                               "(fv2d_set_prim) and read back after (fv2d_get_prim), host clock; not `value`"),
         "hbm_frac_step": round(BYTES_PER_CELL_STEP["fp64"] * cells / (r64["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
@@ -409,6 +420,7 @@ def main(argv=None):
             "ms_per_step": round(r32["ms_per_step"], 4),
             "speedup_vs_fp64": round(r32["gcells"] / r64["gcells"], 3),
             "repetitions": r32["blocks"],
+            "pcie_inclusive": r32["pcie"],
             "hbm_frac_step": round(BYTES_PER_CELL_STEP["fp32"] * cells / (r32["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "roofline": {
                 "bound": "hbm",
