@@ -333,6 +333,9 @@ def main(argv=None):
     if a.calibrate_copy:
         return calibrate_copy(a.calibrate_copy)
 
+    if a.steps < 1 or a.warmup < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported: this build measures one GPU only; the "
               "multi-GPU decomposition is a design (y-slab halo exchange), not a measured path", file=sys.stderr)
