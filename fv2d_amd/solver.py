@@ -95,7 +95,10 @@ class Solver:
         return s.as_tuple()
 
     def advance(self, tend: float = 1e300, max_steps: int = 0, t0: float = 0.0, record_dts: bool = True):
-        """Run the loop on the device; returns (t, dts, counts)."""
+        """Run the loop on the device until t + epsilon >= tend or max_steps steps; returns
+        (t, dts, counts, steps taken). One of the two ends must be given."""
+        if max_steps <= 0 and not tend < 1e300:
+            raise ValueError("advance: give tend or max_steps > 0 (the loop would not end)")
         t = C.c_double(t0)
         n = C.c_int32(0)
         s = StepStats()
