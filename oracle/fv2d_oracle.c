@@ -248,10 +248,22 @@ static void fslp(state qL, state qR, state *flux, double *pout, double gdx, doub
   const double rhoR = qR.v[IR], uR = qR.v[IU], pR = qR.v[IP];
   const double csR = sqrt(gamma0 * pR / rhoR);
 
+#if PLANT(14)
+  const double ai = K * kmin(rhoL * csL, rhoR * csR);
+#else
   const double ai = K * kmax(rhoL * csL, rhoR * csR);
+#endif
+#if PLANT(13)
+  const double theta = kmin(1.0, kmin(fabs(uL) / csL, fabs(uR) / csR));
+#else
   const double theta = kmin(1.0, kmax(fabs(uL) / csL, fabs(uR) / csR));
+#endif
 
+#if PLANT(15)
+  const double ustar = 0.5 * (uR + uL) - 0.5 / ai * (pR - pL + 0.5 * (rhoL + rhoR) * gdx);
+#else
   const double ustar = 0.5 * (uR + uL) - 0.5 / ai * (pR - pL - 0.5 * (rhoL + rhoR) * gdx);
+#endif
   const double Pi = 0.5 * (pR + pL) - theta * 0.5 * ai * (uR - uL);
 
   const state qs = (ustar > 0 ? qL : qR);
@@ -284,7 +296,16 @@ static inline double get_gravity(const orc_sim *s, int i, int j, int dir) {
     /* getAnalyticalGravity (Gravity.h:15-30), hot bubble, also returned as float */
     double x, y;
     get_pos(s, i, j, &x, &y);
+#if PLANT(16)
+    double ga = s->p.hot_bubble_g0 * sin(x * M_PI * 2.0 / s->p.ymax);
+#elif PLANT(18)
+    double ga = s->p.hot_bubble_g0 * sin(y * M_PI * 2.0 / (s->p.ymax - s->p.ymin));
+#else
     double ga = s->p.hot_bubble_g0 * sin(y * M_PI * 2.0 / s->p.ymax);
+#endif
+#if PLANT(17)
+    return ga;
+#endif
     g = (double)(float)ga;
     break;
   }
@@ -474,7 +495,13 @@ static void update_along_dir(orc_sim *s, const double *Q, double *U, double dt, 
   if (s->p.well_balanced_flux_at_y_bc && (j == s->jbeg || j == s->jend - 1) && dir == IY) {
     double g = get_gravity(s, i, j, dir);
     if (j == s->jbeg) {
+#if PLANT(21)
+      state f = {{0.0, 0.0, poutR + aget(s, Q, IR, i, j) * g * s->dy, 0.0}};
+#elif PLANT(22)
+      state f = {{0.0, 0.0, poutL - aget(s, Q, IR, i, j) * g * s->dy, 0.0}};
+#else
       state f = {{0.0, 0.0, poutR - aget(s, Q, IR, i, j) * g * s->dy, 0.0}};
+#endif
       fluxL = f;
     } else {
       state f = {{0.0, 0.0, poutL + aget(s, Q, IR, i, j) * g * s->dy, 0.0}};
@@ -536,9 +563,28 @@ static void apply_thermal_conduction(orc_sim *s, const double *Q, double *U, dou
 
       /* wall modes replace the x fluxes FL / FR on the first / last row
        * (ThermalConduction.h:75-103) */
+#if PLANT(20)
+      /* the physically expected form: the wall modes replace the y fluxes FU / FD */
       if (j == s->jbeg && s->p.bctc_ymin != FV2D_BCTC_NONE) {
         if (s->p.bctc_ymin == FV2D_BCTC_FIXED_TEMPERATURE)
+          FU = kappaU * 2.0 * (TC - s->p.bctc_ymin_value) / dy;
+        else if (s->p.bctc_ymin == FV2D_BCTC_FIXED_GRADIENT)
+          FU = kappaU * s->p.bctc_ymin_value;
+      }
+      if (j == s->jend - 1 && s->p.bctc_ymax != FV2D_BCTC_NONE) {
+        if (s->p.bctc_ymax == FV2D_BCTC_FIXED_TEMPERATURE)
+          FD = kappaD * 2.0 * (s->p.bctc_ymax_value - TC) / dy;
+        else if (s->p.bctc_ymax == FV2D_BCTC_FIXED_GRADIENT)
+          FD = kappaD * s->p.bctc_ymax_value;
+      }
+#else
+      if (j == s->jbeg && s->p.bctc_ymin != FV2D_BCTC_NONE) {
+        if (s->p.bctc_ymin == FV2D_BCTC_FIXED_TEMPERATURE)
+#if PLANT(19)
+          FL = kappaL * (TC - s->p.bctc_ymin_value) / dy;
+#else
           FL = kappaL * 2.0 * (TC - s->p.bctc_ymin_value) / dy;
+#endif
         else if (s->p.bctc_ymin == FV2D_BCTC_FIXED_GRADIENT)
           FL = kappaL * s->p.bctc_ymin_value;
       }
@@ -548,6 +594,7 @@ static void apply_thermal_conduction(orc_sim *s, const double *Q, double *U, dou
         else if (s->p.bctc_ymax == FV2D_BCTC_FIXED_GRADIENT)
           FR = kappaR * s->p.bctc_ymax_value;
       }
+#endif
 
       size_t k = (size_t)IE * s->plane + at(s, i, j);
 #if PLANT(7)

@@ -16,7 +16,8 @@ import math
 import numpy as np
 
 from fv2d_amd import problems as P
-from fv2d_amd.capi import (BC_ABSORBING, BC_PERIODIC, BC_REFLECTING, FSLP, GRAV_CONSTANT, HLL, HLLC,
+from fv2d_amd.capi import (BC_ABSORBING, BC_PERIODIC, BC_REFLECTING, BCTC_FIXED_GRADIENT,
+                           BCTC_FIXED_TEMPERATURE, FSLP, GRAV_ANALYTICAL, GRAV_CONSTANT, HLL, HLLC,
                            PCM, PLM, TS_EULER, TS_RK2, Params)
 from oracle import oracle as OR
 
@@ -408,10 +409,134 @@ def pin_check_negatives(lib):
     o.close()
 
 
+# ------------------------------------------------------------------ FSLP, hot-bubble gravity,
+# wall conduction modes, well-balanced wall flux
+
+def pin_fslp_hydrostatic_pair(lib):
+    """FSLP (Bourgeois et al. 2024, eq. 15): at rest (uL = uR = 0) a pair in discrete
+    hydrostatic balance, pR - pL = (rhoL + rhoR) / 2 * g dx, has u* = 0 exactly, so no mass
+    or energy flux and a momentum flux equal to the interface pressure (pL + pR) / 2.
+    rhoL = 1, rhoR = 3, g dx = -1/4: pR - pL = -1/2."""
+    for K, g0 in ((1.1, 1.4), (1.5, 5.0 / 3.0)):
+        f, pout = OR.riemann("fslp", [1.0, 0.0, 0.3, 2.0], [3.0, 0.0, -0.2, 1.5], g0, lib, fslp_K=K, gdx=-0.25)
+        assert f[0] == 0.0 and f[2] == 0.0 and f[3] == 0.0, (K, f)
+        assert f[1] == 1.75 and pout == 1.75, (K, f, pout)
+    # off balance (g dx of the other sign) the pair moves up the pressure gradient's push:
+    # u* = -(pR - pL - (rhoL+rhoR)/2 g dx) / (2 ai) = -(-1/2 - 1/2) / (2 ai) > 0
+    f, _ = OR.riemann("fslp", [1.0, 0.0, 0.3, 2.0], [3.0, 0.0, -0.2, 1.5], 1.4, lib, fslp_K=1.1, gdx=0.25)
+    assert f[0] > 0.0, f
+
+
+def pin_fslp_hand_value(lib):
+    """FSLP on states whose numbers are exact in binary, worked by hand from eq. 15, 17,
+    31, 32 and 77 of Bourgeois et al. (2024). gamma = 2, K = 2, g dx = 1/2.
+    Left: rho 1, u 1, p 2 -> c = sqrt(2*2/1) = 2, rho c = 2, |u|/c = 1/2.
+    Right: rho 2, u -1, v 1/2, p 16 -> c = sqrt(2*16/2) = 4, rho c = 8, |u|/c = 1/4.
+    a_i = K max(rho c) = 16 (the right state), theta = min(1, max(1/2, 1/4)) = 1/2.
+    u* = (uR + uL)/2 - (1/(2 a_i)) (pR - pL - (rhoL+rhoR)/2 g dx) = 0 - (14 - 3/4)/32
+       = -53/128.
+    Pi = (pR + pL)/2 - theta a_i (uR - uL)/2 = 9 + 8 = 17.
+    u* < 0: upwind state is the right one: E* = 1/2 * 2 * (1 + 1/4) + 16/(2-1) = 69/4.
+    flux = (u* rho, u* rho u + Pi, u* rho v, u* (E* + Pi)) with rho, u, v of the right state."""
+    us = -53.0 / 128.0
+    f, pout = OR.riemann("fslp", [1.0, 1.0, -0.75, 2.0], [2.0, -1.0, 0.5, 16.0], 2.0, lib, fslp_K=2.0, gdx=0.5)
+    exp = [us * 2.0, us * 2.0 * -1.0 + 17.0, us * 2.0 * 0.5, us * (69.0 / 4.0 + 17.0)]
+    assert list(f) == exp and pout == 17.0, (list(f), exp, pout)
+    assert exp == [-0.828125, 17.828125, -0.4140625, -14.181640625]
+    # theta saturates at 1 for a supersonic state: |uL|/cL = 3/2 -> Pi = 9 - 16 (uR - uL) / 2
+    f, pout = OR.riemann("fslp", [1.0, 3.0, 0.0, 2.0], [2.0, 3.0, 0.0, 16.0], 2.0, lib, fslp_K=2.0, gdx=0.0)
+    assert pout == 9.0, pout  # uR - uL = 0
+    f, pout = OR.riemann("fslp", [1.0, 3.0, 0.0, 2.0], [2.0, 2.0, 0.0, 16.0], 2.0, lib, fslp_K=2.0, gdx=0.0)
+    assert pout == 9.0 + 8.0, pout  # theta = 1: Pi = 9 - 1 * 16 * (-1) / 2
+
+
+def pin_hot_bubble_gravity(lib):
+    """getAnalyticalGravity (Gravity.h:16-30): g0 sin(2 pi y / ymax) at the cell centre's y,
+    passed through float. Domain y in [-1, 2] (ymin != 0, and ymax is not the domain
+    length), 6 rows: centres y = -0.75, -0.25, ..., 1.75, where sin(pi y) = +-sqrt(2)/2;
+    x centres 0.5, 1.5 (sin(pi x) = +-1) so that an x in place of y shows."""
+    g0 = 0.3
+    p = Params.defaults(Nx=2, Ny=6, xmin=0.0, xmax=2.0, ymin=-1.0, ymax=2.0, gravity_mode=GRAV_ANALYTICAL,
+                        hot_bubble_g0=g0)
+    o = OR.Oracle(p, lib)
+    h = math.sqrt(2.0) / 2.0
+    signs = [-1, -1, 1, 1, -1, -1]  # sin(pi y) at y = -0.75, -0.25, 0.25, 0.75, 1.25, 1.75
+    for j in range(6):
+        exp = float(np.float32(signs[j] * g0 * h))
+        for i in range(2):
+            for d in (0, 1):
+                got = o.gravity(o.ibeg + i, o.jbeg + j, d)
+                assert got == exp, (i, j, d, got, exp)
+    o.close()
+
+
+def _uniform_rest(Nx, Ny, rho, p):
+    return np.stack([np.full((Ny, Nx), rho), np.zeros((Ny, Nx)), np.zeros((Ny, Nx)), np.full((Ny, Nx), p)])
+
+
+def pin_bctc_wall_modes(lib):
+    """Wall conduction modes (ThermalConduction.h:75-106) on a uniform gas at rest, one Euler
+    step: the hydro fluxes and the interior conduction fluxes vanish, and on the first /
+    last row the mode's flux replaces the x flux FL / FR, so that E changes by
+    -(dt/dx) F_wall / +(dt/dx) F_wall (dx, not dy: the reference writes the x slots).
+    kappa = 1/4, T = p / rho = 2, dx = 1/8, dy = 1/4, dt = 2^-10, numbers exact in binary.
+    fixed_temperature: F = kappa * 2 (T - T_w) / dy; fixed_gradient: F = kappa * G."""
+    dt = 2.0 ** -10
+    cases = [
+        # (mode, ymin value, ymax value, dE row 0, dE row Ny-1)
+        (BCTC_FIXED_TEMPERATURE, 1.5, 3.0, -(dt * 8) * (0.25 * 2 * 0.5 / 0.25), (dt * 8) * (0.25 * 2 * 1.0 / 0.25)),
+        (BCTC_FIXED_GRADIENT, 4.0, -8.0, -(dt * 8) * (0.25 * 4.0), (dt * 8) * (0.25 * -8.0)),
+    ]
+    for mode, vmin, vmax, d0, d1 in cases:
+        assert d0 in (-2.0 ** -7,) and d1 in (2.0 ** -6, -2.0 ** -6)
+        p = Params.defaults(Nx=8, Ny=4, xmax=1.0, ymax=1.0, boundary_x=BC_PERIODIC, boundary_y=BC_REFLECTING,
+                            tc_active=1, kappa=0.25, bctc_ymin=mode, bctc_ymax=mode, bctc_ymin_value=vmin,
+                            bctc_ymax_value=vmax, gamma0=1.4, time_stepping=TS_EULER, reconstruction=PLM)
+        o = OR.Oracle(p, lib)
+        o.set_prim(_uniform_rest(8, 4, 1.0, 2.0))
+        U0 = o.get_cons()
+        o.step(dt)
+        dU = o.get_cons() - U0
+        assert np.all(dU[:3] == 0.0), (mode, dU[:3])
+        assert np.all(dU[3, 1:3] == 0.0), (mode, dU[3])
+        assert np.all(dU[3, 0] == d0) and np.all(dU[3, 3] == d1), (mode, dU[3, 0], dU[3, 3], d0, d1)
+        o.close()
+
+
+def pin_well_balanced_column(lib):
+    """Lake at rest with the well-balanced wall flux (Update.h:149-156) and FSLP: a column
+    of uniform density in discrete hydrostatic balance, p_{j+1} - p_j = rho g dy, with
+    reflecting y walls. Interior faces: u* = 0 exactly (FSLP's gravity term), momentum
+    flux (p_j + p_{j+1}) / 2, whose difference cancels the source dt rho g. Wall rows: the
+    flux is replaced by (0, 0, pout of the other face -+ rho g dy, 0), which cancels it
+    too. So one Euler step leaves U unchanged, bit for bit (numbers exact in binary:
+    g = -1/2, dy = 1/4, p = 2, 15/8, 7/4, 13/8, dt = 2^-10). Without the wall flux the
+    wall rows move."""
+    ps = [2.0, 1.875, 1.75, 1.625]
+    q = np.stack([np.ones((4, 2)), np.zeros((4, 2)), np.zeros((4, 2)), np.array(ps)[:, None] * np.ones((4, 2))])
+    for wb in (1, 0):
+        p = Params.defaults(Nx=2, Ny=4, xmax=1.0, ymax=1.0, boundary_x=BC_PERIODIC, boundary_y=BC_REFLECTING,
+                            riemann_solver=FSLP, reconstruction=PCM, time_stepping=TS_EULER, gamma0=1.4,
+                            gravity_mode=GRAV_CONSTANT, gx=0.0, gy=-0.5, well_balanced_flux_at_y_bc=wb)
+        o = OR.Oracle(p, lib)
+        o.set_prim(q)
+        U0 = o.get_cons()
+        o.step(2.0 ** -10)
+        dU = o.get_cons() - U0
+        if wb:
+            assert np.all(dU == 0.0), dU
+        else:
+            assert np.all(dU[:, 1:3] == 0.0), dU  # interior rows in balance
+            assert np.all(dU[2, 0] != 0.0) and np.all(dU[2, 3] != 0.0), dU
+        o.close()
+
+
 ALL_PINS = [
     pin_prim_cons_hand_values, pin_prim_cons_roundtrip, pin_sound_speed, pin_minmod_table,
     pin_riemann_consistency, pin_riemann_supersonic, pin_hllc_stationary_contact, pin_riemann_mirror_wall,
     pin_riemann_vs_exact, pin_boundaries, pin_dt_closed_forms, pin_gravity_float, pin_free_stream,
     pin_y_uniform_sod, pin_sodx_transpose_sody, pin_periodic_conservation, pin_reflecting_conservation,
     pin_sod_convergence, pin_tc_fourier, pin_visc_shear_wave, pin_gravity_energy, pin_check_negatives,
+    pin_fslp_hydrostatic_pair, pin_fslp_hand_value, pin_hot_bubble_gravity, pin_bctc_wall_modes,
+    pin_well_balanced_column,
 ]

@@ -155,3 +155,48 @@ def test_negative_reset_counts(oracle_lib, gpu_lib):
         seen += sum(co)
         np.testing.assert_array_equal(g.get_prim(), o.get_prim())
     assert seen > 0, "the case did not trigger the reset"
+
+
+def test_closed_form_setups_on_gpu(gpu_lib):
+    """The closed-form setups that pin the oracle's FSLP, wall-flux and wall-conduction
+    code, run through the HIP path: a lake at rest with FSLP and the well-balanced wall
+    flux stays bit for bit at rest, and the wall conduction modes change E on the wall rows
+    by exactly -(dt/dx) F_wall / +(dt/dx) F_wall (numbers exact in binary, derivations in
+    tests/oracle_pins.py: pin_well_balanced_column, pin_bctc_wall_modes)."""
+    from fv2d_amd.capi import BCTC_FIXED_GRADIENT, BCTC_FIXED_TEMPERATURE, TS_EULER
+
+    ps = [2.0, 1.875, 1.75, 1.625]
+    q = np.stack([np.ones((4, 2)), np.zeros((4, 2)), np.zeros((4, 2)), np.array(ps)[:, None] * np.ones((4, 2))])
+    p = Params.defaults(Nx=2, Ny=4, xmax=1.0, ymax=1.0, boundary_x=BC_PERIODIC, boundary_y=BC_REFLECTING,
+                        riemann_solver=FSLP, reconstruction=PCM, time_stepping=TS_EULER, gamma0=1.4,
+                        gravity_mode=GRAV_CONSTANT, gx=0.0, gy=-0.5, well_balanced_flux_at_y_bc=1)
+    with Solver(p, gpu_lib) as g:
+        g.set_prim(q)
+        U0 = g.get_cons()
+        g.step(2.0 ** -10)
+        assert np.all(g.get_cons() - U0 == 0.0)
+    dt = 2.0 ** -10
+    for mode, vmin, vmax, d0, d1 in ((BCTC_FIXED_TEMPERATURE, 1.5, 3.0, -2.0 ** -7, 2.0 ** -6),
+                                     (BCTC_FIXED_GRADIENT, 4.0, -8.0, -2.0 ** -7, -2.0 ** -6)):
+        p = Params.defaults(Nx=8, Ny=4, xmax=1.0, ymax=1.0, boundary_x=BC_PERIODIC, boundary_y=BC_REFLECTING,
+                            tc_active=1, kappa=0.25, bctc_ymin=mode, bctc_ymax=mode, bctc_ymin_value=vmin,
+                            bctc_ymax_value=vmax, gamma0=1.4, time_stepping=TS_EULER, reconstruction=PLM)
+        q = np.stack([np.ones((4, 8)), np.zeros((4, 8)), np.zeros((4, 8)), np.full((4, 8), 2.0)])
+        with Solver(p, gpu_lib) as g:
+            g.set_prim(q)
+            U0 = g.get_cons()
+            g.step(dt)
+            dU = g.get_cons() - U0
+        assert np.all(dU[:3] == 0.0) and np.all(dU[3, 1:3] == 0.0), mode
+        assert np.all(dU[3, 0] == d0) and np.all(dU[3, 3] == d1), (mode, dU[3, 0], dU[3, 3])
+
+
+def test_hot_bubble_rows_on_gpu(oracle_lib, gpu_lib):
+    """Analytical (hot-bubble) gravity on a grid with ymin != 0, against the oracle's
+    closed-form-pinned gravity (pin_hot_bubble_gravity), through a whole RK2 run."""
+    p = Params.defaults(Nx=20, Ny=36, xmin=0.0, xmax=2.0, ymin=-1.0, ymax=2.0, boundary_x=BC_PERIODIC,
+                        boundary_y=BC_REFLECTING, reconstruction=PLM, riemann_solver=HLLC, time_stepping=1,
+                        gravity_mode=GRAV_ANALYTICAL, hot_bubble_g0=0.3, well_balanced_flux_at_y_bc=1,
+                        gamma0=1.4, CFL=0.5)
+    q0 = P.random_state(p, seed=77)
+    assert run_pair(p, q0, 6, oracle_lib, gpu_lib)

@@ -19,6 +19,16 @@ PLANTS = {
     10: "x and y slopes swapped",
     11: "RK2 averages with U* instead of U^n",
     12: "hyperbolic dt uses dx for the y term",
+    13: "FSLP theta takes the smaller of |u|/c over the two states",
+    14: "FSLP a_i takes the smaller acoustic impedance",
+    15: "FSLP gravity term of u* with the wrong sign",
+    16: "hot-bubble gravity evaluated at x instead of y",
+    17: "hot-bubble gravity not passed through float",
+    18: "hot-bubble gravity divides by the domain length instead of ymax",
+    19: "fixed-temperature wall flux without the factor 2",
+    20: "wall conduction modes replace the y fluxes FU / FD instead of FL / FR",
+    21: "well-balanced wall flux with the wrong sign of rho g dy",
+    22: "well-balanced bottom wall flux takes the pout of its own face",
 }
 
 
