@@ -10,8 +10,8 @@ synthetic seeded initial state, all state resident in HBM. One step = one full R
 (dt reduction + two fused stage launches). The FP64 parity form gives `value`
 (Gcell-updates/s = Nx*Ny*K / time of K steps, HIP events on the solver stream, device
 synchronised on both sides); the FP32 form is timed the same way and reported beside it.
-The K steps run as --blocks consecutive device loops with an event between them:
-`repetitions` gives the median, minimum and maximum rate of the blocks. `pcie_inclusive`
+The K steps run as one device loop (fv2d_advance); `repetitions` gives the median, minimum
+and maximum rate of --reps further loops of K steps each (when K >= 32). `pcie_inclusive`
 adds the upload of the host primitives before and their read-back after (host clock).
 
 roofline: the dominant kernel is the fused stage kernel (two launches per step). Its
@@ -70,7 +70,7 @@ def read_traffic(prec: str, size: int):
     return float(m.group(1)) if m else None
 
 
-def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, blocks: int = 5):
+def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps: int = 5):
     from fv2d_amd import hiprt
     from fv2d_amd import problems as P
     from fv2d_amd.capi import FP32, FP64
@@ -88,21 +88,13 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, block
     stream = s.lib.fv2d_get_stream(s.h)
     hiprt.synchronize()
     s.lib.fv2d_profile(s.h, 1)
-    # the K timed steps run as `blocks` consecutive device loops with an event between them:
-    # the total gives `value`, the blocks give the median / spread of repetitions
-    nb = max(1, min(blocks, steps))
-    bounds = [steps * k // nb for k in range(nb + 1)]
-    ev = [hiprt.Event() for _ in range(nb + 1)]
+    # the K timed steps: one device loop, HIP events on the solver stream around it
+    ev = [hiprt.Event() for _ in range(2)]
     ev[0].record(stream)
-    t, n = 0.0, 0
-    for k in range(nb):
-        t, _, counts, taken = s.advance(max_steps=bounds[k + 1] - bounds[k], t0=t, record_dts=False)
-        n += taken
-        ev[k + 1].record(stream)
+    t, _, counts, n = s.advance(max_steps=steps, t0=0.0, record_dts=False)
+    ev[1].record(stream)
     hiprt.synchronize()
-    ms = ev[0].elapsed_ms(ev[-1])
-    block_rates = [p.Nx * p.Ny * (bounds[k + 1] - bounds[k]) / (ev[k].elapsed_ms(ev[k + 1]) * 1e-3) / 1e9
-                   for k in range(nb)]
+    ms = ev[0].elapsed_ms(ev[1])
     import ctypes as C
 
     st_ms = C.c_double()
@@ -111,6 +103,16 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, block
     s.lib.fv2d_profile(s.h, 0)
     assert n == steps, f"advance took {n} steps, expected {steps}"
     cells = p.Nx * p.Ny
+    # repetitions (spread): `reps` more device loops of the same K steps, each event-timed;
+    # only when a loop is long enough (>= 32 steps) for its rate to mean something
+    rep_rates = []
+    if steps >= 32:
+        for _ in range(reps):
+            ev[0].record(stream)
+            t, _, _, nr = s.advance(max_steps=steps, t0=t, record_dts=False)
+            ev[1].record(stream)
+            hiprt.synchronize()
+            rep_rates.append(cells * nr / (ev[0].elapsed_ms(ev[1]) * 1e-3) / 1e9)
     gcells = cells * steps / (ms * 1e-3) / 1e9
     mean_launch_s = st_ms.value * 1e-3 / max(st_n.value, 1)
     bpl = BYTES_PER_CELL_STEP[prec] / STAGE_LAUNCHES_PER_STEP
@@ -120,13 +122,15 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, block
     t_down = time.perf_counter() - t_down
     sums = s.sums()
     s.close()
+    reps_out = {"n": len(rep_rates), "steps_each": steps}
+    if rep_rates:
+        reps_out.update(gcells_median=round(float(np.median(rep_rates)), 4), gcells_min=round(min(rep_rates), 4),
+                        gcells_max=round(max(rep_rates), 4))
     return dict(gcells=gcells, ms_per_step=ms / steps, mean_launch_ms=mean_launch_s * 1e3, launches=int(st_n.value),
                 achieved=achieved, bytes_per_launch=bpl * cells, counts=counts, q=q, sums=sums,
                 pcie={"upload_ms": round(t_up * 1e3, 2), "download_ms": round(t_down * 1e3, 2),
                       "gcells": round(cells * steps / (ms * 1e-3 + t_up + t_down) / 1e9, 4)},
-                blocks={"n": nb, "steps": [bounds[k + 1] - bounds[k] for k in range(nb)],
-                        "gcells_median": round(float(np.median(block_rates)), 4),
-                        "gcells_min": round(min(block_rates), 4), "gcells_max": round(max(block_rates), 4)})
+                blocks=reps_out)
 
 
 def run_cpu(size: int, seconds: float, q0: np.ndarray):
@@ -310,8 +314,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--blocks", type=int, default=5,
-                    help="the timed steps run as this many consecutive blocks, each event-timed (median reported)")
+    ap.add_argument("--reps", type=int, default=5,
+                    help="further event-timed device loops of K steps for the spread (median reported; K >= 32)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -333,8 +337,8 @@ def main(argv=None):
     if a.calibrate_copy:
         return calibrate_copy(a.calibrate_copy)
 
-    if a.steps < 1 or a.warmup < 0:
-        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+    if a.steps < 1 or a.warmup < 0 or a.serve_steps < 1 or a.serve_jobs < 1:
+        print("bench.py: --steps, --serve-steps and --serve-jobs must be >= 1 and --warmup >= 0", file=sys.stderr)
         return 2
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported: this build measures one GPU only; the "
@@ -353,12 +357,12 @@ def main(argv=None):
     q0 = P.kh_ic(P.kh_params(N=a.size))
 
     log(f"bench: FP64 form, {a.warmup} warmup + {a.steps} timed steps")
-    r64 = run_gpu("fp64", a.size, a.steps, a.warmup, q0, a.blocks)
+    r64 = run_gpu("fp64", a.size, a.steps, a.warmup, q0, a.reps)
     log(f"bench: FP64 {r64['gcells']:.3f} Gcell/s, {r64['ms_per_step']:.3f} ms/step")
     r32 = None
     if not a.no_fp32:
         log(f"bench: FP32 form, {a.warmup} warmup + {a.steps} timed steps")
-        r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0, a.blocks)
+        r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0, a.reps)
         log(f"bench: FP32 {r32['gcells']:.3f} Gcell/s, {r32['ms_per_step']:.3f} ms/step")
     fp32_err = None
     if not a.no_fp32 and not a.no_errors and a.size == 4096:

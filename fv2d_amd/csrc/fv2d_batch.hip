@@ -487,6 +487,8 @@ int fv2d_batch_submit(fv2d_batch *b, const fv2d_params *p, const fv2d_problem *p
     BFAIL(b, FV2D_E_INVALID, "fv2d_batch_submit: job precision %d differs from the batch's %d", p->precision,
           b->cfg.precision);
   if (!(tend == tend)) BFAIL(b, FV2D_E_INVALID, "fv2d_batch_submit: tend is NaN");
+  if (max_steps <= 0 && !(tend < FV2D_TEND_NONE))
+    BFAIL(b, FV2D_E_INVALID, "fv2d_batch_submit: no end: give tend < 1e300 or max_steps > 0");
   JobHost J;
   J.p = *p;
   const int pitch = (p->Nx + 63) / 64 * 64;

@@ -101,6 +101,16 @@ inline void build_kparams(const fv2d_params *p, int pitch, KParams &kp) {
   kp.epsilon = p->epsilon;
 }
 
+// getAnalyticalGravity (Gravity.h:15-30) for rows [j0, j1): it depends on the row only; it is
+// evaluated with the host libm and passed through float, as getGravity returns float.
+inline void gravity_rows(const fv2d_params &p, const KParams &kp, int j0, int j1, float *out) {
+  for (int j = j0; j < j1; ++j) {
+    const double y = p.ymin + (j + 0.5) * kp.dy;
+    const double g = p.hot_bubble_g0 * sin(y * M_PI * 2.0 / p.ymax);
+    out[j - j0] = (float)g;
+  }
+}
+
 // Rows per workgroup segment: 64 by default (a 4-row prologue per segment, enough
 // workgroups to fill 256 CUs at 4096^2), fewer on short grids so that small grids still
 // spread over the chip.

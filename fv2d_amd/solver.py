@@ -170,6 +170,8 @@ class Batch:
 
     def submit(self, params: Params, problem: Problem | None = None, q: np.ndarray | None = None,
                tend: float = 1e300, max_steps: int = 0) -> int:
+        if max_steps <= 0 and not tend < 1e300:
+            raise ValueError("submit: give tend or max_steps > 0 (the job would not end)")
         jid = C.c_int64()
         qa = None
         if q is not None:

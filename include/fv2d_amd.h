@@ -40,6 +40,10 @@ extern "C" {
 #define FV2D_E_STATE (-5)       /* call made in the wrong order (e.g. step before set_prim) */
 #define FV2D_E_BUSY (-6)        /* batch: no free slot or arena space for the job right now */
 
+/* A loop end time at or above this value (or NaN) counts as "no end time": fv2d_advance and
+ * fv2d_batch_submit then need max_steps > 0 and return FV2D_E_INVALID otherwise. */
+#define FV2D_TEND_NONE 1e300
+
 /* Option values: same members and order as the reference enums (SimInfo.h:41-97). */
 enum fv2d_riemann { FV2D_HLL = 0, FV2D_HLLC = 1, FV2D_FSLP = 2 };
 enum fv2d_boundary { FV2D_BC_ABSORBING = 0, FV2D_BC_REFLECTING = 1, FV2D_BC_PERIODIC = 2 };
@@ -146,10 +150,11 @@ int fv2d_step(fv2d_handle *h, double dt, fv2d_step_stats *stats);
 
 /* Replaces: the time loop of main.cpp:62-84 without I/O. Runs steps with the dt kept on the
  * device (dt_n = computeDt(Q_n), t += dt_n) while t + epsilon < tend and fewer than
- * max_steps steps were taken in this call (max_steps <= 0: no step limit). Writes the dt of
- * every step into dts (capacity max_steps entries; may be NULL), the number of steps into
- * *nsteps and the accumulated reset counters into stats (may be NULL). *t is the start time
- * on entry and the end time on return. */
+ * max_steps steps were taken in this call (max_steps <= 0: no step limit). A call with
+ * neither end (max_steps <= 0 and tend >= FV2D_TEND_NONE or NaN) returns FV2D_E_INVALID.
+ * Writes the dt of every step into dts (capacity max_steps entries; may be NULL), the number
+ * of steps into *nsteps and the accumulated reset counters into stats (may be NULL). *t is
+ * the start time on entry and the end time on return. */
 int fv2d_advance(fv2d_handle *h, double *t, double tend, int32_t max_steps, double *dts,
                  int32_t *nsteps, fv2d_step_stats *stats);
 
@@ -257,7 +262,8 @@ const char *fv2d_batch_last_error(const fv2d_batch *b);
 /* Admit a job: parameters p (precision must be the batch's), initial state from prob
  * (device-generated) or from q (4*Nx*Ny primitives, snapshot order; exactly one of the two
  * non-NULL), run until t + epsilon >= tend or max_steps steps (<= 0: no limit), the loop
- * rule of main.cpp:62-84. FV2D_E_BUSY if no slot or arena space is free now. */
+ * rule of main.cpp:62-84; a job with neither end (max_steps <= 0 and tend >= FV2D_TEND_NONE or
+ * NaN) returns FV2D_E_INVALID. FV2D_E_BUSY if no slot or arena space is free now. */
 int fv2d_batch_submit(fv2d_batch *b, const fv2d_params *p, const fv2d_problem *prob, const double *q,
                       double tend, int32_t max_steps, int64_t *job_id);
 

@@ -27,6 +27,13 @@ def test_single_handle_order_and_arguments(gpu_lib):
         assert gpu_lib.fv2d_set_problem(s.h, C.byref(Problem(kind=9))) == capi.FV2D_E_INVALID
         t = C.c_double(0.0)
         assert gpu_lib.fv2d_advance(s.h, None, 1.0, 1, None, None, None) == capi.FV2D_E_INVALID
+        # a loop with neither an end time nor a step limit is refused (it would not end)
+        s.set_problem(capi.PROB_SOD_X)
+        for tend in (float("inf"), 1e300, float("nan")):
+            assert gpu_lib.fv2d_advance(s.h, C.byref(t), tend, 0, None, None, None) == capi.FV2D_E_INVALID
+            assert b"no end" in gpu_lib.fv2d_last_error(s.h)
+        with pytest.raises(ValueError):
+            s.advance()
         # still usable
         s.set_problem(capi.PROB_SOD_X)
         s.advance(max_steps=3)
@@ -46,6 +53,11 @@ def test_batch_submit_errors(gpu_lib):
         rc = gpu_lib.fv2d_batch_submit(b.b, C.byref(p), C.byref(S.job_problem(sp)),
                                        q.ctypes.data_as(C.POINTER(C.c_double)), 1e300, 4, C.byref(jid))
         assert rc == capi.FV2D_E_INVALID  # both prob and q
+        for tend in (float("inf"), 1e300):  # neither an end time nor a step limit
+            rc = gpu_lib.fv2d_batch_submit(b.b, C.byref(p), C.byref(S.job_problem(sp)), None, tend, 0, C.byref(jid))
+            assert rc == capi.FV2D_E_INVALID and b"no end" in gpu_lib.fv2d_batch_last_error(b.b)
+        with pytest.raises(ValueError):
+            b.submit(p, problem=S.job_problem(sp))
         with pytest.raises(Fv2dError):
             b.submit(p.copy(Nx=1), problem=S.job_problem(sp), max_steps=4)  # invalid parameters
         j = b.submit(p, problem=S.job_problem(sp), max_steps=4)
