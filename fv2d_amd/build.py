@@ -37,6 +37,7 @@ UNITS = [
     ("stage_fp32.hip", ["-ffp-contract=fast"]),
     ("fv2d_capi.hip", ["-ffp-contract=off"]),
     ("fv2d_batch.hip", ["-ffp-contract=off"]),
+    ("fv2d_noalloc.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -249,6 +249,8 @@ def declare(lib: C.CDLL) -> None:
         "fv2d_batch_collect": (C.c_int, [H, C.c_int64, dp, dp, P(JobInfo)]),
         "fv2d_batch_cancel": (C.c_int, [H, C.c_int64]),
         "fv2d_batch_last_launch_ms": (C.c_int, [H, dp]),
+        "fv2d_run_noalloc": (C.c_int, [P(Params), dp, dp, dp, C.c_double, C.c_int32, dp, P(C.c_int32),
+                                       P(StepStats), C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -289,4 +291,5 @@ EXPORTED_SYMBOLS = (
     "fv2d_batch_collect",
     "fv2d_batch_cancel",
     "fv2d_batch_last_launch_ms",
+    "fv2d_run_noalloc",
 )

@@ -128,7 +128,8 @@ int fv2d_create(const fv2d_params *p, fv2d_handle **h);
 /* Frees the handle and its device memory. NULL is allowed. */
 void fv2d_destroy(fv2d_handle *h);
 
-/* Message of the last failing call on h (h == NULL: the last failing create/check). */
+/* Message of the last failing call on h (h == NULL: the last failing create / check /
+ * fv2d_run_noalloc of the calling thread). */
 const char *fv2d_last_error(const fv2d_handle *h);
 
 /* Replaces: the primitive state produced by InitFunctor::init (Init.h:310-358) or
@@ -289,6 +290,38 @@ int fv2d_batch_cancel(fv2d_batch *b, int64_t id);
 
 /* Duration (ms) of the last waited-for launches, from HIP events on the batch stream. */
 int fv2d_batch_last_launch_ms(fv2d_batch *b, double *ms);
+
+/* ------------------------------------------------------------------------------------
+ * Runs that allocate nothing.
+ *
+ * No counterpart in the reference. For callers that cannot allocate device memory during
+ * a run: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc. fv2d_run_noalloc does what fv2d_create + fv2d_set_prim + fv2d_advance +
+ * fv2d_get_prim + fv2d_destroy do, with the same kernels and bit for bit the same results,
+ * but calls no hipMalloc / hipFree: the simulation's buffers (state, stage state, initial
+ * primitives, control block, dt partials, dt log, gravity rows) live in a fixed device
+ * array of the library (about 96 MiB plus 0.3 MiB) that every GPU the function runs on
+ * holds until the process ends.
+ *
+ * q_in, q_out: DEVICE pointers, 4*Nx*Ny doubles each in snapshot order (the caller
+ * allocates them); q_out may equal q_in. *t: start time on entry, end time on return.
+ * tend, max_steps, dts (host, capacity max_steps), nsteps, stats: as fv2d_advance (stats
+ * counts this run). stream: the hipStream_t to run on (NULL: the null stream); the call
+ * returns when its work is done.
+ *
+ * Limits (FV2D_E_INVALID beyond them): row pitch (Nx rounded up to 64) times Ny at most
+ * FV2D_NOALLOC_MAX_PLANE cells, Ny at most FV2D_NOALLOC_MAX_ROWS, max_steps at most
+ * FV2D_NOALLOC_MAX_STEPS, at most FV2D_NOALLOC_MAX_WORKGROUPS stage workgroups.
+ * Concurrency: calls on one GPU share its array; calls from threads of one process are
+ * serialised by the library, calls that overlapped on one GPU would race.
+ * ------------------------------------------------------------------------------------ */
+#define FV2D_NOALLOC_MAX_PLANE (1 << 20)
+#define FV2D_NOALLOC_MAX_ROWS 16384
+#define FV2D_NOALLOC_MAX_STEPS 16384
+#define FV2D_NOALLOC_MAX_WORKGROUPS 16384
+
+int fv2d_run_noalloc(const fv2d_params *p, const double *q_in, double *q_out, double *t, double tend,
+                     int32_t max_steps, double *dts, int32_t *nsteps, fv2d_step_stats *stats, void *stream);
 
 #ifdef __cplusplus
 }
