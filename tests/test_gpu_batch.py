@@ -305,3 +305,53 @@ def test_random_stream_fp64_each_job_equals_single_path(gpu_lib):
         q, inf = got[k]
         assert inf.nsteps == na and inf.t == ta
         assert np.array_equal(q, qa), f"job {k} ({S.KIND_NAMES[sp.kind]} {sp.Nx}x{sp.Ny}) differs from the single path"
+
+
+# The served stream's grid sizes (1024^2, 2048x512, 1024x256, 512^2) mixed with 128^2 jobs.
+STREAM_MIX = [spec(PROB_KH, 1024, 1024, 5 / 3, 6e-3, 21, steps=40), spec(PROB_BLAST, 2048, 512, 1.45, steps=35),
+              spec(PROB_SOD_X, 1024, 256, 1.6, steps=30), spec(PROB_KH, 512, 512, 1.5, 9e-3, 4, steps=45),
+              spec(PROB_KH, 128, 128, 1.55, 3e-3, 8, steps=50), spec(PROB_BLAST, 128, 128, 1.62, steps=25),
+              spec(PROB_SOD_X, 128, 128, 1.41, steps=33), spec(PROB_KH, 512, 512, 1.4, 2e-3, 17, steps=20)]
+
+
+def test_stream_sizes_mix_equals_alone_fp32(gpu_lib):
+    """Jobs at the served stream's real sizes (1024^2, 2048x512, 1024x256, 512^2) in one
+    batch with 128^2 jobs, half of them admitted two launches after the others (the FP32
+    form, as the stream runs): every job bitwise equal to the same job run alone through the
+    single-simulation path. Tolerance: zero."""
+    cfg = BatchConfig.defaults(precision=FP32, steps_per_launch=8, max_jobs=16, arena_bytes=2 << 30)
+    jobs = [(S.job_params(sp, FP32), S.job_problem(sp), None, sp.steps) for sp in STREAM_MIX]
+    got = run_batch(jobs, FP32, gpu_lib, cfg=cfg, stagger=4)
+    for k, sp in enumerate(STREAM_MIX):
+        qa, sa, ta, na = alone(sp, FP32, gpu_lib)
+        q, sums, inf = got[k]
+        assert inf.nsteps == na == sp.steps, (k, inf.nsteps, na)
+        assert inf.t == ta, f"job {k}: t {inf.t!r} != alone {ta!r}"
+        assert np.array_equal(q, qa), f"job {k} ({S.KIND_NAMES[sp.kind]} {sp.Nx}x{sp.Ny}): not bitwise equal alone"
+        assert np.array_equal(sums, sa)
+
+
+def test_stream_size_fp64_job_equals_oracle(oracle_lib, gpu_lib):
+    """One FP64 512^2 Kelvin-Helmholtz job of the stream in a batch with FP64 jobs of the
+    other stream sizes: bitwise equal to the oracle from the same initial state (the device
+    initial condition read back before the run), step by step dt included."""
+    sp = spec(PROB_KH, 512, 512, 1.5, 9e-3, 4, steps=30)
+    p = S.job_params(sp, FP64)
+    with Solver(p, gpu_lib) as s:  # the device-generated initial state, as the batch job gets it
+        s.set_problem(sp.kind, sp.seed, sp.amplitude)
+        q0 = s.get_prim()
+    others = [spec(PROB_BLAST, 1024, 256, 1.45, steps=12), spec(PROB_SOD_X, 128, 128, 1.6, steps=40),
+              spec(PROB_KH, 1024, 1024, 1.4, 2e-3, 3, steps=8)]
+    jobs = [(p, S.job_problem(sp), None, sp.steps)] + [(S.job_params(o, FP64), S.job_problem(o), None, o.steps)
+                                                        for o in others]
+    cfg = BatchConfig.defaults(precision=FP64, steps_per_launch=8, max_jobs=8, arena_bytes=2 << 30)
+    got = run_batch(jobs, FP64, gpu_lib, cfg=cfg, stagger=2)
+    o = Oracle(p, oracle_lib)
+    o.set_prim(q0)
+    t_o, dts_o, _ = o.run(1e300, sp.steps)
+    qo = o.get_prim()
+    o.close()
+    q, _, inf = got[0]
+    assert inf.nsteps == len(dts_o) == sp.steps
+    assert inf.t == t_o, (inf.t, t_o)
+    assert np.array_equal(q, qo), f"stream-size FP64 job differs from the oracle: rel Linf {rel_linf(q, qo):.3e}"
