@@ -20,7 +20,7 @@ PCM, PCM_WB, PLM = 0, 1, 2
 GRAV_NONE, GRAV_CONSTANT, GRAV_ANALYTICAL = 0, 1, 2
 TCM_CONSTANT, TCM_B02 = 0, 1
 BCTC_NONE, BCTC_FIXED_TEMPERATURE, BCTC_FIXED_GRADIENT = 0, 1, 2
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_STATE64 = 0, 1, 2
 
 FV2D_OK = 0
 FV2D_E_INVALID = -1
@@ -71,7 +71,8 @@ class Params(C.Structure):
         ("epsilon_reset_negative", C.c_double),
         ("precision", C.c_int32),
         ("seg_rows", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("chunk_rows", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
     @classmethod

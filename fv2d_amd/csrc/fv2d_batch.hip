@@ -147,18 +147,26 @@ struct JobLayout {
 
 size_t align_up(size_t x) { return (x + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
 
+// Rows per chunk of a batched job: its own chunk_rows, else 4.
+int batch_chunk_rows(const fv2d_params *p) { return p->chunk_rows > 0 ? p->chunk_rows : CHUNK_ROWS_DEFAULT; }
+
 int batch_seg_rows(const fv2d_params *p) {
   if (p->seg_rows > 0) return p->seg_rows;
   // jobs share the chip: full-height segments (4-row prologue per 64 rows) unless the grid
   // is shorter
-  const int s = (p->Ny + 3) / 4 * 4;
+  const int nw = batch_chunk_rows(p);
+  const int s = (p->Ny + nw - 1) / nw * nw;
   return s < 64 ? s : 64;
+}
+
+StripGeom batch_geom(const fv2d_params *p) {
+  return strip_geom(p->Nx, p->Ny, batch_seg_rows(p), batch_chunk_rows(p));
 }
 
 JobLayout job_layout(const fv2d_params *p) {
   const size_t esize = (p->precision == FV2D_FP32) ? 4 : 8;
   const size_t pitch = (size_t)(p->Nx + 63) / 64 * 64;
-  const StripGeom g = strip_geom(p->Nx, p->Ny, batch_seg_rows(p));
+  const StripGeom g = batch_geom(p);
   JobLayout L;
   L.state = align_up(4 * pitch * (size_t)p->Ny * esize);
   L.partials = align_up(sizeof(double) * (size_t)g.nblocks);
@@ -174,6 +182,7 @@ StageClass class_of(const fv2d_params &p, const KParams &kp) {
   c.tc = p.tc_active ? 1 : 0;
   c.visc = p.visc_active ? 1 : 0;
   c.xg = needs_xg(kp) ? 1 : 0;
+  c.nw = batch_chunk_rows(&p);
   return c;
 }
 
@@ -493,7 +502,7 @@ int fv2d_batch_submit(fv2d_batch *b, const fv2d_params *p, const fv2d_problem *p
   J.p = *p;
   const int pitch = (p->Nx + 63) / 64 * 64;
   build_kparams(p, pitch, J.kp);
-  J.g = strip_geom(p->Nx, p->Ny, batch_seg_rows(p));
+  J.g = batch_geom(p);
   ProblemArgs pa;
   if (prob && problem_args(*p, J.kp, *prob, pa))
     BFAIL(b, FV2D_E_INVALID, "fv2d_batch_submit: unknown problem kind %d or bad amplitude", prob->kind);

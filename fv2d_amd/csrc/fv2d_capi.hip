@@ -173,7 +173,7 @@ int fv2d_create(const fv2d_params *p, fv2d_handle **out) {
   h->esize = (p->precision == FV2D_FP32) ? 4 : 8;
   const int pitch = (p->Nx + 63) / 64 * 64;
   build_kparams(p, pitch, h->d.kp);
-  h->d.geom = strip_geom(p->Nx, p->Ny, pick_seg_rows(p));
+  h->d.geom = sim_geom(p);
   h->d.nparts = h->d.geom.nblocks;
 
   auto fail = [&](const char *what, hipError_t e) {
@@ -330,7 +330,9 @@ int fv2d_advance(fv2d_handle *h, double *t, double tend, int32_t max_steps, doub
   if (max_steps <= 0 && !(tend < FV2D_TEND_NONE))
     FAILH(h, FV2D_E_INVALID, "fv2d_advance: no end: give tend < 1e300 or max_steps > 0");
   HIPH(h, hipSetDevice(h->device));
-  if (max_steps > h->dts_cap) {
+  // the device log of the dts exists only when the caller asks for them (no allocation
+  // otherwise)
+  if (dts && max_steps > h->dts_cap) {
     if (h->dts_log) HIPH(h, hipFree(h->dts_log));
     h->dts_log = nullptr;
     h->dts_cap = 0;
@@ -338,7 +340,7 @@ int fv2d_advance(fv2d_handle *h, double *t, double tend, int32_t max_steps, doub
     h->dts_cap = max_steps;
   }
   StepCtl c;
-  SIMH(h, sim_advance(h->d, *t, tend, max_steps, h->dts_log, c));
+  SIMH(h, sim_advance(h->d, *t, tend, max_steps, dts ? h->dts_log : nullptr, c));
   const int done = c.nsteps;
   if (dts && done > 0) {
     HIPH(h, hipMemcpyAsync(dts, h->dts_log, sizeof(double) * done, hipMemcpyDeviceToHost, h->d.stream));

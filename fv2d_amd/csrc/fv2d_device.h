@@ -86,10 +86,10 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
   __syncthreads();
 
   int xw = 0;
-  for (int jc = jb; jc < je; jc += STRIP_ROWS) {
+  for (int jc = jb; jc < je; jc += K::NW) {
     if (K::kPF) {
       K::chunk_commit(lane, w, xw, i0, jb, jc, sm, kp, io, pf, r);
-      if (jc + STRIP_ROWS < je) K::chunk_prefetch(lane, w, (xw + 1) & (STRIP_ROWS - 1), i0, jc + STRIP_ROWS, kp, io, pf);
+      if (jc + K::NW < je) K::chunk_prefetch(lane, w, (xw + 1) % K::NW, i0, jc + K::NW, kp, io, pf);
     } else {
       K::chunk_load(lane, w, xw, i0, jb, jc, sm, kp, io, r);
     }
@@ -114,7 +114,7 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
     __syncthreads();
     K::update_cell(lane, w, i0, jb, jc, je, sm, kp, io, dt, dt_dx, dt_dy, r);
     __syncthreads();
-    xw = (xw + 1) & (STRIP_ROWS - 1);
+    xw = (xw + 1) % K::NW;
   }
   if (io.epilogue) commit_epilogue<K::NT>(r.neg_rho, r.neg_prs, r.nan_cnt, r.inv, io.ctl, io.partials, blk);
 }
@@ -184,32 +184,33 @@ batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restric
     g.nsy = J.nsy;
     g.seg = J.seg;
     g.nblocks = J.nblocks;
+    g.nw = K::NW;
     strip_block<K, R, S>(J.kp, io, g, wi.blk, uni(ctl->dt), uni(ctl->dt_dx), uni(ctl->dt_dy), sm);
     __syncthreads();
   }
 }
 
-// Kernel class of a simulation: the template switches of the stage.
+// Kernel class of a simulation: the template switches of the stage and the chunk height.
 struct StageClass {
-  int recon, riem, tc, visc, xg;
+  int recon, riem, tc, visc, xg, nw;
   bool operator==(const StageClass &o) const {
-    return recon == o.recon && riem == o.riem && tc == o.tc && visc == o.visc && xg == o.xg;
+    return recon == o.recon && riem == o.riem && tc == o.tc && visc == o.visc && xg == o.xg && nw == o.nw;
   }
 };
 
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW>
 hipError_t launch_batch_x(const BatchJob *jobs, const BatchItem *items, const int *range, int *counter, int stage,
                           int grid, hipStream_t st) {
-  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG> K;
+  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW> K;
   hipLaunchKernelGGL((batch_stage_kernel<K, R, S>), dim3(grid), dim3(K::NT), 0, st, jobs, items, range, counter,
                      stage);
   return hipGetLastError();
 }
 
 // Resident workgroups per CU of a class's batched kernel (for the persistent grid size).
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW>
 int batch_occupancy_x() {
-  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG> K;
+  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW> K;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_stage_kernel<K, R, S>, K::NT, 0) != hipSuccess) n = 1;
   return n > 0 ? n : 1;
@@ -247,11 +248,23 @@ auto with_class(const StageClass &c, F &&f) {
 #undef FV2D_CLS
 }
 
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
-hipError_t launch_stage_x(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
-  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG> K;
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW>
+hipError_t launch_stage_n(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
+  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW> K;
   hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
   return hipGetLastError();
+}
+
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+hipError_t launch_stage_x(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
+  switch (g.nw) {
+  case 4:
+    return launch_stage_n<R, S, RECON, RIEM, TC, VISC, XG, 4>(kp, io, g, st);
+  case 8:
+    return launch_stage_n<R, S, RECON, RIEM, TC, VISC, XG, 8>(kp, io, g, st);
+  default:
+    return hipErrorInvalidValue;
+  }
 }
 
 // Gravity, the well-balanced wall flux and wall conduction modes need the XG variant.
@@ -306,5 +319,7 @@ hipError_t launch_stage_fp64(const KParams &kp, const StageIO<double> &io, const
                              bool tc, bool visc, hipStream_t st);
 hipError_t launch_stage_fp32(const KParams &kp, const StageIO<float> &io, const StripGeom &g, int recon, int riem,
                              bool tc, bool visc, hipStream_t st);
+hipError_t launch_stage_fp32s64(const KParams &kp, const StageIO<double> &io, const StripGeom &g, int recon,
+                                int riem, bool tc, bool visc, hipStream_t st);
 
 }  // namespace fv2d

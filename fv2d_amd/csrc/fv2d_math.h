@@ -36,16 +36,32 @@ struct St {
 template <typename R>
 struct Ops;
 
-// a / d for a divisor d fixed over the grid (dx or dy), given rd = RN(1/d).
-// Markstein's correction (P. Markstein, IBM J. Res. Dev. 34(1), 1990): with q0 = RN(a*rd)
-// within one ulp of a/d, r = a - d*q0 is exact in one FMA and RN(q0 + r*rd) is the
-// correctly rounded quotient, barring underflow and overflow. The fast path is taken only
-// for 2^-960 <= |a| <= 2^960 and when the host checked 2^-60 <= d <= 2^60 (fast != 0), so
-// every intermediate stays normal; zeros return a itself (a / d keeps the sign of a);
-// anything else (tiny, huge, inf, NaN) uses the IEEE division.
-FV2D_HD double markstein_div(double a, double d, double rd, int fast) {
+// a / d for a divisor d fixed over the grid (dx or dy), given rd = RN(1/d), in one of three
+// modes chosen on the host by div_mode(d):
+//  DIV_EXACT (2): d is a power of two whose reciprocal is a normal double, so rd = 1/d
+//    exactly and a * rd is the correctly rounded a / d for every a (both round the same
+//    real number; zeros, infinities and NaNs included): one multiply.
+//  DIV_MARKSTEIN (1): Markstein's correction (P. Markstein, IBM J. Res. Dev. 34(1), 1990):
+//    with q0 = RN(a*rd) within one ulp of a/d, r = a - d*q0 is exact in one FMA and
+//    RN(q0 + r*rd) is the correctly rounded quotient, barring underflow and overflow. The
+//    fast path is taken only for 2^-960 <= |a| <= 2^960 and 2^-60 <= d <= 2^60, so every
+//    intermediate stays normal; zeros return a itself (a / d keeps the sign of a); anything
+//    else (tiny, huge, inf, NaN) uses the IEEE division.
+//  DIV_IEEE (0): the IEEE division.
+enum : int { DIV_IEEE = 0, DIV_MARKSTEIN = 1, DIV_EXACT = 2 };
+
+FV2D_HD int div_mode(double d) {
+  if (d >= 0x1p-1022 && d <= 0x1p+1022) {
+    int e;
+    if (frexp(d, &e) == 0.5) return DIV_EXACT;
+  }
+  return (d >= 0x1p-60 && d <= 0x1p+60) ? DIV_MARKSTEIN : DIV_IEEE;
+}
+
+FV2D_HD double markstein_div(double a, double d, double rd, int mode) {
+  if (mode == DIV_EXACT) return a * rd;
   const double aa = fabs(a);
-  if (fast) {
+  if (mode == DIV_MARKSTEIN) {
     if (aa >= 0x1p-960 && aa <= 0x1p+960) {
       const double q0 = a * rd;
       const double r = fma(-d, q0, a);
@@ -81,7 +97,7 @@ FV2D_HD double sqrt_ieee(double x) {
 template <>
 struct Ops<double> {
   static FV2D_HD double div(double a, double b) { return a / b; }
-  static FV2D_HD double div_c(double a, double d, double rd, int fast) { return markstein_div(a, d, rd, fast); }
+  static FV2D_HD double div_c(double a, double d, double rd, int mode) { return markstein_div(a, d, rd, mode); }
   static FV2D_HD double sqrt_(double a) { return sqrt_ieee(a); }
   static FV2D_HD double min_(double a, double b) { return fmin(a, b); }
   static FV2D_HD double max_(double a, double b) { return fmax(a, b); }
@@ -96,9 +112,9 @@ struct Ops<float> {
     return a / b;
 #endif
   }
-  static FV2D_HD float div_c(float a, float d, float rd, int fast) {
+  static FV2D_HD float div_c(float a, float d, float rd, int mode) {
     (void)d;
-    (void)fast;
+    (void)mode;
     return a * rd;
   }
   static FV2D_HD float sqrt_(float a) {
@@ -168,7 +184,7 @@ struct KParams {
   int grav;           // gravity mode
   int wb;             // well_balanced_flux_at_y_bc
   int bctc_ymin, bctc_ymax;
-  int fastdiv;        // dx and dy allow the Markstein division (markstein_div)
+  int divx, divy;     // markstein_div mode of the divisions by dx and by dy (div_mode)
   long long plane;    // elements per field plane (pitch * Ny)
   double gamma0, gm1, entho;
   double dx, dy, inv_dx, inv_dy;  // inv_* = 1.0 / d (Viscosity.h:57-58), also RN(1/d) for markstein_div
@@ -306,18 +322,17 @@ FV2D_HD void hll(const St<R> &qL, const St<R> &qR, R gamma0, R gm1, St<R> &flux,
   }
 }
 
-// hllc (RiemannSolvers.h:53-128). Only the star state of the selected branch is divided
-// out; the value of the selected branch is the reference's bit for bit.
+// hllc (RiemannSolvers.h:53-128). The reference evaluates both star states and both total
+// energies and then picks one of four branches (SL > 0, uS > 0, SR > 0, else, at
+// RiemannSolvers.h:93-122); here the side X of the picked branch is found first (the left
+// side when SL > 0 or uS > 0), and only that side's total energy and, in the star branches,
+// its star state are computed, by the same expressions on the same operands: the value of
+// the picked branch is the reference's bit for bit.
 template <typename R>
 FV2D_HD void hllc(const St<R> &qL, const St<R> &qR, R gamma0, R entho, St<R> &flux, R &pout) {
   typedef Ops<R> O;
   const R rL = qL.v[IR], uL = qL.v[IU], vL = qL.v[IV], pL = qL.v[IP];
   const R rR = qR.v[IR], uR = qR.v[IU], vR = qR.v[IV], pR = qR.v[IP];
-
-  const R ekL = R(0.5) * rL * (uL * uL + vL * vL);
-  const R EL = ekL + pL * entho;
-  const R ekR = R(0.5) * rR * (uR * uR + vR * vR);
-  const R ER = ekR + pR * entho;
 
   const R cfastL = speed_of_sound(qL, gamma0);
   const R cfastR = speed_of_sound(qR, gamma0);
@@ -334,53 +349,34 @@ FV2D_HD void hllc(const St<R> &qL, const St<R> &qR, R gamma0, R entho, St<R> &fl
   const R uS = byd.first();
   const R pS = byd(rcR * pL + rcL * pR + rcL * rcR * (uL - uR));
 
-  // Branches SL > 0, uS > 0, SR > 0, else (RiemannSolvers.h:93-122)
-  R sr, su, sv, sp, E;
-  if (SL > R(0)) {
-    sr = rL;
-    su = uL;
-    sv = vL;
-    sp = pL;
-    E = EL;
-    pout = pL;
-  } else if (SR > R(0)) {
-    // star region: left star if uS > 0, right star otherwise
-    const bool left = uS > R(0);
-    const R rX = left ? rL : rR;
-    const R uX = left ? uL : uR;
-    const R pX = left ? pL : pR;
-    const R EX = left ? EL : ER;
+  // branches SL > 0 (left state), uS > 0 (left star), SR > 0 (right star), else (right state)
+  const bool left = (SL > R(0)) || (uS > R(0));
+  const bool star = !(SL > R(0)) && ((uS > R(0)) || (SR > R(0)));
+  const R rX = left ? rL : rR;
+  const R uX = left ? uL : uR;
+  const R vX = left ? vL : vR;
+  const R pX = left ? pL : pR;
+  // EL / ER (RiemannSolvers.h:71-74) of the picked side
+  const R EX = R(0.5) * rX * (uX * uX + vX * vX) + pX * entho;
+  R sr, su, sp, E;
+  if (star) {
     const R SX = left ? SL : SR;
-    const R dS = SX - uS;
-    const DivBy<R> bys(dS, rX * (SX - uX));
+    const DivBy<R> bys(SX - uS, rX * (SX - uX));
     sr = bys.first();
     E = bys((SX - uX) * EX - pX * uX + pS * uS);
     su = uS;
-    sv = left ? vL : vR;
     sp = pS;
-    pout = pS;
-  } else if (uS > R(0)) {
-    // SR <= 0 < uS cannot happen for ordered speeds, but the reference tests uS first
-    const R dS = SL - uS;
-    const DivBy<R> bys(dS, rL * (SL - uL));
-    sr = bys.first();
-    E = bys((SL - uL) * EL - pL * uL + pS * uS);
-    su = uS;
-    sv = vL;
-    sp = pS;
-    pout = pS;
   } else {
-    sr = rR;
-    su = uR;
-    sv = vR;
-    sp = pR;
-    E = ER;
-    pout = pR;
+    sr = rX;
+    su = uX;
+    sp = pX;
+    E = EX;
   }
+  pout = sp;
 
   flux.v[IR] = sr * su;
   flux.v[IU] = sr * su * su + sp;
-  flux.v[IV] = flux.v[IR] * sv;
+  flux.v[IV] = flux.v[IR] * vX;
   flux.v[IE] = (E + sp) * su;
 }
 
@@ -456,8 +452,8 @@ FV2D_HD int map_ghost(int g, int N, int bc, bool &flip) {
 template <typename R>
 FV2D_HD double inv_dt_hyp_cs(const St<R> &q, R cs, const KParams &kp) {
   if (sizeof(R) == 8) {
-    return markstein_div((double)(cs + fabs(q.v[IU])), kp.dx, kp.inv_dx, kp.fastdiv) +
-           markstein_div((double)(cs + fabs(q.v[IV])), kp.dy, kp.inv_dy, kp.fastdiv);
+    return markstein_div((double)(cs + fabs(q.v[IU])), kp.dx, kp.inv_dx, kp.divx) +
+           markstein_div((double)(cs + fabs(q.v[IV])), kp.dy, kp.inv_dy, kp.divy);
   } else {
     // FP32 form: FP64 for the reduction, per-cell value from the FP32 state
     return ((double)cs + fabs((double)q.v[IU])) * kp.inv_dx + ((double)cs + fabs((double)q.v[IV])) * kp.inv_dy;

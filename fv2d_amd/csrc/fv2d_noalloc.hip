@@ -106,7 +106,7 @@ int fv2d_run_noalloc(const fv2d_params *p, const double *q_in, double *q_out, do
   SimDev d;
   d.p = *p;
   build_kparams(p, pitch, d.kp);
-  d.geom = strip_geom(p->Nx, p->Ny, pick_seg_rows(p));
+  d.geom = sim_geom(p);
   d.nparts = d.geom.nblocks;
   if (d.nparts > FV2D_NOALLOC_MAX_WORKGROUPS)
     return fail(FV2D_E_INVALID, "fv2d_run_noalloc: %d workgroups exceed the partials capacity %d (raise seg_rows)",

@@ -9,7 +9,7 @@
 #include <string>
 
 #include "../../include/fv2d_amd.h"
-#include "fv2d_math.h"
+#include "fv2d_stage.h"
 
 namespace fv2d {
 
@@ -41,12 +41,17 @@ inline int check_params(const fv2d_params *p, std::string &why) {
   if (p->reconstruction < 0 || p->reconstruction > 2 || p->riemann_solver < 0 || p->riemann_solver > 2 ||
       p->time_stepping < 0 || p->time_stepping > 1 || p->boundary_x < 0 || p->boundary_x > 2 || p->boundary_y < 0 ||
       p->boundary_y > 2 || p->gravity_mode < 0 || p->gravity_mode > 2 || p->bctc_ymin < 0 || p->bctc_ymin > 2 ||
-      p->bctc_ymax < 0 || p->bctc_ymax > 2 || p->precision < 0 || p->precision > 1) {
+      p->bctc_ymax < 0 || p->bctc_ymax > 2 || p->precision < 0 || p->precision > 2) {
     why = "an option value is out of range";
     return FV2D_E_INVALID;
   }
-  if (p->seg_rows < 0 || p->seg_rows % 4 != 0) {
-    snprintf(b, sizeof b, "seg_rows=%d must be 0 or a positive multiple of 4", p->seg_rows);
+  if (p->chunk_rows != 0 && !chunk_rows_supported(p->chunk_rows)) {
+    snprintf(b, sizeof b, "chunk_rows=%d must be 0, 4 or 8", p->chunk_rows);
+    why = b;
+    return FV2D_E_INVALID;
+  }
+  if (p->seg_rows != 0 && p->seg_rows < 4) {
+    snprintf(b, sizeof b, "seg_rows=%d must be 0 or at least 4", p->seg_rows);
     why = b;
     return FV2D_E_INVALID;
   }
@@ -81,7 +86,8 @@ inline void build_kparams(const fv2d_params *p, int pitch, KParams &kp) {
   kp.dy = (p->ymax - p->ymin) / p->Ny;
   kp.inv_dx = 1.0 / kp.dx;
   kp.inv_dy = 1.0 / kp.dy;
-  kp.fastdiv = (kp.dx >= 0x1p-60 && kp.dx <= 0x1p+60 && kp.dy >= 0x1p-60 && kp.dy <= 0x1p+60) ? 1 : 0;
+  kp.divx = div_mode(kp.dx);
+  kp.divy = div_mode(kp.dy);
   kp.gxf = (double)(float)p->gx;
   kp.gyf = (double)(float)p->gy;
   kp.gdx_x = p->gx * kp.dx;
@@ -111,15 +117,27 @@ inline void gravity_rows(const fv2d_params &p, const KParams &kp, int j0, int j1
   }
 }
 
-// Rows per workgroup segment: 64 by default (a 4-row prologue per segment, enough
-// workgroups to fill 256 CUs at 4096^2), fewer on short grids so that small grids still
-// spread over the chip.
-inline int pick_seg_rows(const fv2d_params *p) {
+// Rows per chunk (= waves per workgroup) of the single-grid stage kernel.
+inline int pick_chunk_rows(const fv2d_params *p) {
+  return p->chunk_rows > 0 ? p->chunk_rows : CHUNK_ROWS_DEFAULT;
+}
+
+// Rows per workgroup segment: about 64 by default, a multiple of the chunk height (a 4-row
+// prologue per segment, enough workgroups to fill 256 CUs at 4096^2), fewer on short grids so
+// that small grids still spread over the chip.
+inline int pick_seg_rows(const fv2d_params *p, int nw) {
   if (p->seg_rows > 0) return p->seg_rows;
-  int seg = 64;
+  const int step = nw < 8 ? 8 : nw;
+  int seg = 64 / step * step;
   const long long strips = (p->Nx + 61) / 62;
-  while (seg > 8 && strips * ((p->Ny + seg - 1) / seg) < 2048) seg -= 8;
+  while (seg > step && strips * ((p->Ny + seg - 1) / seg) < 2048) seg -= step;
   return seg;
+}
+
+// Strip geometry of the single-grid kernel.
+inline StripGeom sim_geom(const fv2d_params *p) {
+  const int nw = pick_chunk_rows(p);
+  return strip_geom(p->Nx, p->Ny, pick_seg_rows(p, nw), nw);
 }
 
 }  // namespace fv2d

@@ -33,7 +33,10 @@ struct SimDev {
 // Sets the message fv2d_last_error(NULL) returns (fv2d_capi.hip).
 void set_global_error(const char *msg);
 
+// The state is stored in FP32 (FV2D_FP32), else in FP64.
 inline bool sim_fp32(const SimDev &d) { return d.p.precision == FV2D_FP32; }
+// FP32 arithmetic on an FP64 state (FV2D_FP32_STATE64).
+inline bool sim_fp32s64(const SimDev &d) { return d.p.precision == FV2D_FP32_STATE64; }
 
 // Max hyperbolic inverse dt of Q0 (from_q0) or of reset(consToPrim(src)) into the partials.
 inline hipError_t sim_launch_dt_tiles(SimDev &d, const void *src, int from_q0) {
@@ -41,6 +44,9 @@ inline hipError_t sim_launch_dt_tiles(SimDev &d, const void *src, int from_q0) {
   if (sim_fp32(d))
     hipLaunchKernelGGL((dt_cells_kernel<float, float>), grid, dim3(256), 0, d.stream, d.kp, (const float *)src, from_q0,
                        d.partials, d.nparts);
+  else if (sim_fp32s64(d))
+    hipLaunchKernelGGL((dt_cells_kernel<float, double>), grid, dim3(256), 0, d.stream, d.kp, (const double *)src,
+                       from_q0, d.partials, d.nparts);
   else
     hipLaunchKernelGGL((dt_cells_kernel<double, double>), grid, dim3(256), 0, d.stream, d.kp, (const double *)src,
                        from_q0, d.partials, d.nparts);
@@ -82,8 +88,12 @@ inline hipError_t sim_launch_stage(SimDev &d, const void *src, int qsrc, const v
     io.partials = d.partials;
     io.qsrc = qsrc;
     io.epilogue = epilogue;
-    e = launch_stage_fp64(d.kp, io, d.geom, recon, d.p.riemann_solver, d.p.tc_active != 0, d.p.visc_active != 0,
-                          d.stream);
+    if (sim_fp32s64(d))
+      e = launch_stage_fp32s64(d.kp, io, d.geom, recon, d.p.riemann_solver, d.p.tc_active != 0,
+                               d.p.visc_active != 0, d.stream);
+    else
+      e = launch_stage_fp64(d.kp, io, d.geom, recon, d.p.riemann_solver, d.p.tc_active != 0,
+                            d.p.visc_active != 0, d.stream);
   }
   if (e != hipSuccess) return e;
   if (d.mark && (e = d.mark(d.mark_ctx)) != hipSuccess) return e;

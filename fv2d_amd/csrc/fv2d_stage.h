@@ -84,17 +84,24 @@ struct BatchItem {
 // Strip / segment geometry shared by the launcher, the kernel and the emulation.
 struct StripGeom {
   int nsx, nsy;  // strips across x, segments along y
-  int seg;       // rows per segment (multiple of 4)
+  int seg;       // rows per segment (a multiple of nw keeps every chunk full)
   int nblocks;
+  int nw;        // rows per chunk = waves per workgroup (a CHUNK_ROWS value)
 };
 
 constexpr int STRIP_OUT = 62;  // output columns per strip
 constexpr int STRIP_W = 64;    // lanes = strip columns held in registers
-constexpr int STRIP_ROWS = 4;  // rows per chunk = waves per workgroup
 
-FV2D_HD StripGeom strip_geom(int Nx, int Ny, int seg) {
+// Rows per chunk (= waves per workgroup) the kernels are built for: 4 or 8. Larger chunks
+// spread the rings' fixed rows over more waves (LDS: FP64 6.2 KB per wave + 13.6 KB per
+// workgroup) and halve the barriers per row, but every barrier then waits for more waves.
+constexpr int CHUNK_ROWS_DEFAULT = 4;
+FV2D_HD bool chunk_rows_supported(int nw) { return nw == 4 || nw == 8; }
+
+FV2D_HD StripGeom strip_geom(int Nx, int Ny, int seg, int nw) {
   StripGeom g;
   g.seg = seg;
+  g.nw = nw;
   g.nsx = (Nx + STRIP_OUT - 1) / STRIP_OUT;
   g.nsy = (Ny + seg - 1) / seg;
   g.nblocks = g.nsx * g.nsy;
@@ -142,14 +149,17 @@ struct LaneRegs {
 
 // XG: gravity, the well-balanced wall flux or wall conduction modes are on; without it
 // their code is compiled out.
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+// NW_: rows per chunk = waves per workgroup (a chunk_rows_supported value).
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW_>
 struct StripStage {
-  static constexpr int NW = STRIP_ROWS;
+  static constexpr int NW = NW_;
   static constexpr int NT = 64 * NW;
   static constexpr int HX = STRIP_W + 2;  // loaded columns: 64 lanes + 2 right halo columns
-  static constexpr int NQ = 7;            // primitive / temperature ring rows
-  static constexpr int NY = 6;            // y-slope / y-flux ring rows
-  static constexpr int NV = 5;            // viscous y-face ring rows
+  // Ring rows. While chunk jc runs (rows jc .. jc+NW-1): primitives of rows jc-1 .. jc+NW+1,
+  // slopes / fluxes of rows jc-1 .. jc+NW, viscous terms of faces jc-1 .. jc+NW-1.
+  static constexpr int NQ = NW + 3;  // primitive / temperature ring rows
+  static constexpr int NY = NW + 2;  // y-slope / y-flux ring rows
+  static constexpr int NV = NW + 1;  // viscous y-face ring rows
   static constexpr bool kTC = TC, kVISC = VISC;
   // Waves per SIMD asked of the compiler (its VGPR budget). FP64: 4 (at most 128 VGPRs; the
   // 39 KB LDS image allows 4 workgroups of 4 waves per CU). FP32: 5 (96 VGPRs); the
@@ -243,7 +253,8 @@ struct StripStage {
   }
 
   // Load one strip row j into the ring: lane l -> strip column l; the two right halo
-  // columns 64, 65 of `nextra` rows starting at j go to lanes 0 .. 2*nextra-1 of wave `xw`.
+  // columns 64, 65 of `nextra` rows starting at j go to lanes 0 .. 2*nextra-1 of wave `xw`
+  // (nextra <= 32).
   static FV2D_HD void load_row(int lane, int w, int j, int i0, int jb, Smem &sm, const KParams &kp,
                                const StageIO<S> &io) {
     St<R> q;
@@ -308,9 +319,9 @@ struct StripStage {
   }
 
   // ---------------------------------------------------------------- segment prologue
-  // Rows jb-2 .. jb+1 into the ring (wave w: row jb-2+w; wave 0 also the right halo).
+  // Rows jb-2 .. jb+1 into the ring (wave w < 4: row jb-2+w; wave 0 also the right halo).
   static FV2D_HD void pro_load(int lane, int w, int i0, int jb, Smem &sm, const KParams &kp, const StageIO<S> &io) {
-    load_row(lane, w, jb - 2 + w, i0, jb, sm, kp, io);
+    if (w < 4) load_row(lane, w, jb - 2 + w, i0, jb, sm, kp, io);
     if (w == 0) load_extra(lane, jb - 2, 4, i0, jb, sm, kp, io);
   }
   // y slopes of rows jb-1 (wave 0) and jb (wave 1).
@@ -364,11 +375,11 @@ struct StripStage {
   }
 
   // ---------------------------------------------------------------- chunk phases
-  // Rows jc+2 .. jc+5 (wave w: jc+2+w); the right halo columns by wave `xw`.
+  // Rows jc+2 .. jc+NW+1 (wave w: jc+2+w); the right halo columns by wave `xw`.
   static FV2D_HD void chunk_load(int lane, int w, int xw, int i0, int jb, int jc, Smem &sm, const KParams &kp,
                                  const StageIO<S> &io, LaneRegs<R> &r) {
     load_row(lane, w, jc + 2 + w, i0, jb, sm, kp, io);
-    if (w == xw) load_extra(lane, jc + 2, 4, i0, jb, sm, kp, io);
+    if (w == xw) load_extra(lane, jc + 2, NW, i0, jb, sm, kp, io);
     load_own(lane, jc + w, i0, kp, io, r);
   }
 
@@ -379,7 +390,7 @@ struct StripStage {
     bool fx, fy;
     fetch_raw(i0 - 2 + lane, jn + 2 + w, kp, io, p.row, fx, fy);
     p.flags = (fx ? 1 : 0) | (fy ? 2 : 0);
-    if (kPFX && w == xwn && lane < 8) {
+    if (kPFX && w == xwn && lane < 2 * NW) {
       fetch_raw(i0 - 2 + STRIP_W + (lane & 1), jn + 2 + (lane >> 1), kp, io, p.extra, fx, fy);
       p.flags |= (fx ? 4 : 0) | (fy ? 8 : 0);
     }
@@ -392,12 +403,12 @@ struct StripStage {
     put(sm, qslot(jc + 2 + w, jb), lane, q, t);
     if (w == xw) {
       if (kPFX) {
-        if (lane < 8) {
+        if (lane < 2 * NW) {
           raw_to_q(p.extra, (p.flags & 4) != 0, (p.flags & 8) != 0, kp, io, q, t);
           put(sm, qslot(jc + 2 + (lane >> 1), jb), STRIP_W + (lane & 1), q, t);
         }
       } else {
-        load_extra(lane, jc + 2, 4, i0, jb, sm, kp, io);
+        load_extra(lane, jc + 2, NW, i0, jb, sm, kp, io);
       }
     }
     load_own(lane, jc + w, i0, kp, io, r);
@@ -457,7 +468,7 @@ struct StripStage {
     if (TC) {
       const R kc = R(kp.kappa);
       const R kf = R(0.5) * (kc + kc);
-      r.tcx = Ops<R>::div_c(kf * (T(sm, j, lane + 1, jb) - T(sm, j, lane, jb)), R(kp.dx), R(kp.inv_dx), kp.fastdiv);
+      r.tcx = Ops<R>::div_c(kf * (T(sm, j, lane + 1, jb) - T(sm, j, lane, jb)), R(kp.dx), R(kp.inv_dx), kp.divx);
     }
     if (VISC) {
       const R mu = R(kp.mu), odx = R(kp.inv_dx), ody = R(kp.inv_dy);
@@ -489,11 +500,11 @@ struct StripStage {
 
   // U += ((FL - FR) * dt) / d (Update.h:158-159); FP32 form: (FL - FR) * (dt / d).
   static FV2D_HD void add_flux(A acc[4], const R FL[4], const R FR[4], double dt, double dt_d, double d, double rd,
-                               int fastdiv) {
+                               int mode) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       if (sizeof(R) == 8)
-        acc[v] = acc[v] + A(Ops<double>::div_c((double)(FL[v] - FR[v]) * dt, d, rd, fastdiv));
+        acc[v] = acc[v] + A(Ops<double>::div_c((double)(FL[v] - FR[v]) * dt, d, rd, mode));
       else
         acc[v] = acc[v] + A(FL[v] - FR[v]) * A(dt_d);
     }
@@ -529,7 +540,7 @@ struct StripStage {
     }
 
     // x sweep (Update.h:171)
-    add_flux(acc, r.fxl, r.fx, dt, dt_dx, kp.dx, kp.inv_dx, kp.fastdiv);
+    add_flux(acc, r.fxl, r.fx, dt, dt_dx, kp.dx, kp.inv_dx, kp.divx);
     if (XG && kp.grav != GRAV_NONE) add_gravity(acc, rho, r.fxl[IR], r.fx[IR], dt, gX);
 
     // y sweep (Update.h:172)
@@ -555,7 +566,7 @@ struct StripStage {
           FR[3] = R(0);
         }
       }
-      add_flux(acc, FL, FR, dt, dt_dy, kp.dy, kp.inv_dy, kp.fastdiv);
+      add_flux(acc, FL, FR, dt, dt_dy, kp.dy, kp.inv_dy, kp.divy);
       if (XG && kp.grav != GRAV_NONE) add_gravity(acc, rho, FL[IR], FR[IR], dt, gY);
     }
 
@@ -567,18 +578,18 @@ struct StripStage {
       const R Tc = T(sm, j, lane, jb);
       R FL = r.tcxl;
       R FR = r.tcx;
-      const R FU = Ops<R>::div_c(kf * (Tc - T(sm, j - 1, lane, jb)), dy, R(kp.inv_dy), kp.fastdiv);
-      const R FD = Ops<R>::div_c(kf * (T(sm, j + 1, lane, jb) - Tc), dy, R(kp.inv_dy), kp.fastdiv);
+      const R FU = Ops<R>::div_c(kf * (Tc - T(sm, j - 1, lane, jb)), dy, R(kp.inv_dy), kp.divy);
+      const R FD = Ops<R>::div_c(kf * (T(sm, j + 1, lane, jb) - Tc), dy, R(kp.inv_dy), kp.divy);
       // wall modes replace the x fluxes on the first / last row (ThermalConduction.h:75-103)
       if (XG && j == 0 && kp.bctc_ymin != BCTC_NONE) {
         if (kp.bctc_ymin == BCTC_FIXED_TEMPERATURE)
-          FL = Ops<R>::div_c(kf * R(2) * (Tc - R(kp.bctc_ymin_value)), dy, R(kp.inv_dy), kp.fastdiv);
+          FL = Ops<R>::div_c(kf * R(2) * (Tc - R(kp.bctc_ymin_value)), dy, R(kp.inv_dy), kp.divy);
         else if (kp.bctc_ymin == BCTC_FIXED_GRADIENT)
           FL = kf * R(kp.bctc_ymin_value);
       }
       if (XG && j == kp.Ny - 1 && kp.bctc_ymax != BCTC_NONE) {
         if (kp.bctc_ymax == BCTC_FIXED_TEMPERATURE)
-          FR = Ops<R>::div_c(kf * R(2) * (R(kp.bctc_ymax_value) - Tc), dy, R(kp.inv_dy), kp.fastdiv);
+          FR = Ops<R>::div_c(kf * R(2) * (R(kp.bctc_ymax_value) - Tc), dy, R(kp.inv_dy), kp.divy);
         else if (kp.bctc_ymax == BCTC_FIXED_GRADIENT)
           FR = kf * R(kp.bctc_ymax_value);
       }

@@ -18,27 +18,31 @@ struct BatchLaunch {
   int *counter;
   int stage, grid;
   hipStream_t st;
+  int nw;
   template <int RC, int RM, bool TC, bool VI, bool XG>
   hipError_t operator()() {
-    return launch_batch_x<double, double, RC, RM, TC, VI, XG>(jobs, items, range, counter, stage, grid, st);
+    if (nw == 8) return launch_batch_x<double, double, RC, RM, TC, VI, XG, 8>(jobs, items, range, counter, stage, grid, st);
+    return launch_batch_x<double, double, RC, RM, TC, VI, XG, 4>(jobs, items, range, counter, stage, grid, st);
   }
 };
 struct BatchOcc {
+  int nw;
   template <int RC, int RM, bool TC, bool VI, bool XG>
   int operator()() {
-    return batch_occupancy_x<double, double, RC, RM, TC, VI, XG>();
+    if (nw == 8) return batch_occupancy_x<double, double, RC, RM, TC, VI, XG, 8>();
+    return batch_occupancy_x<double, double, RC, RM, TC, VI, XG, 4>();
   }
 };
 }  // namespace
 
 hipError_t launch_batch_fp64(const StageClass &c, const BatchJob *jobs, const BatchItem *items, const int *range,
                              int *counter, int stage, int grid, hipStream_t st) {
-  BatchLaunch f{jobs, items, range, counter, stage, grid, st};
+  BatchLaunch f{jobs, items, range, counter, stage, grid, st, c.nw};
   return with_class(c, f);
 }
 
 int batch_occupancy_fp64(const StageClass &c) {
-  BatchOcc f;
+  BatchOcc f{c.nw};
   return with_class(c, f);
 }
 

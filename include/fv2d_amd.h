@@ -55,8 +55,10 @@ enum fv2d_bctc_mode { FV2D_BCTC_NONE = 0, FV2D_BCTC_FIXED_TEMPERATURE = 1, FV2D_
 /* Arithmetic form. FP64 is the parity form. FP32 runs reconstruction, Riemann fluxes and the
  * parabolic/gravity stencils in FP32 and stores the state in FP32; the step's increment dt*dF
  * is formed in FP32 and added to U in FP64 (with the RK2 average), and the conservation sums
- * and the dt reduction are FP64. */
-enum fv2d_precision { FV2D_FP64 = 0, FV2D_FP32 = 1 };
+ * and the dt reduction are FP64. FP32_STATE64 is the FP32 form with the state U stored in
+ * FP64 (the increment is added to the FP64 state, which is never rounded to FP32): twice the
+ * state bytes of FP32, the FP32 form's arithmetic. Batches take FP64 or FP32. */
+enum fv2d_precision { FV2D_FP64 = 0, FV2D_FP32 = 1, FV2D_FP32_STATE64 = 2 };
 
 /*
  * Solver configuration: the hot-path subset of the reference's DeviceParams/Params
@@ -98,8 +100,9 @@ typedef struct fv2d_params {
   double epsilon_reset_negative; /* value written over negative density / pressure */
   /* this build */
   int32_t precision; /* fv2d_precision */
-  int32_t seg_rows;  /* rows per workgroup segment of the stage kernel, multiple of 4 (0: automatic) */
-  int32_t reserved[6];
+  int32_t seg_rows;   /* rows per workgroup segment of the stage kernel, >= 4 (0: automatic) */
+  int32_t chunk_rows; /* rows per chunk = waves per workgroup of the stage kernel: 4 or 8 (0: automatic) */
+  int32_t reserved[5];
 } fv2d_params;
 
 /* Per-step counters of the negative-value reset (SimInfo.h:604-606). */

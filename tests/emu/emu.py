@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def build() -> None:
-    subprocess.run(["make", "-s", "-j2", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-j8", "-C", HERE], check=True)
 
 
 def load(sanitized: bool = False):
@@ -34,7 +34,9 @@ def load(sanitized: bool = False):
     lib.emu_get_prim.argtypes = [C.c_void_p, dp]
     lib.emu_get_cons.argtypes = [C.c_void_p, dp]
     lib.emu_markstein_mismatches.restype = C.c_longlong
-    lib.emu_markstein_mismatches.argtypes = [dp, C.c_longlong, C.c_double]
+    lib.emu_markstein_mismatches.argtypes = [dp, C.c_longlong, C.c_double, C.c_int]
+    lib.emu_div_mode.restype = C.c_int
+    lib.emu_div_mode.argtypes = [C.c_double]
     return lib
 
 

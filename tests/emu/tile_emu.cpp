@@ -8,145 +8,12 @@
 // use, so a read of anything not written first shows up in the results. Built with
 // -fsanitize=address,undefined it also checks every global and LDS index. Used by the CPU
 // test suite to compare the kernel logic with the oracle before any GPU run.
-#include <math.h>
-#include <stdint.h>
-#include <stdlib.h>
-#include <string.h>
-
-#include <limits>
-#include <string>
-#include <vector>
-
-#include "../../fv2d_amd/csrc/fv2d_params.h"
-#include "../../fv2d_amd/csrc/fv2d_stage.h"
+#include "tile_emu.h"
 
 using namespace fv2d;
+using namespace emu;
 
 namespace {
-
-struct Sim {
-  fv2d_params p;
-  KParams kp;
-  bool fp32;
-  std::vector<double> Ud, Usd, Q0d;
-  std::vector<float> Uf, Usf, Q0f;
-  std::vector<float> grav_row;
-  std::vector<double> partials;
-  StripGeom g;
-  StepCtl ctl;
-  bool fresh = false;
-  unsigned long long last[3] = {0, 0, 0};
-};
-
-// Lane l receives lane l-1's value; lane 0 gets 0 (the DPP wave_shr:1 of the kernel).
-template <typename R, typename F>
-void shift(std::vector<LaneRegs<R>> &regs, F field) {
-  for (int w = 0; w < STRIP_ROWS; ++w)
-    for (int l = 63; l >= 0; --l) {
-      LaneRegs<R> &dst = regs[w * 64 + l];
-      field(dst, l ? &regs[w * 64 + l - 1] : nullptr);
-    }
-}
-
-template <class K, typename S>
-void run_stage(Sim &s, const StageIO<S> &io) {
-  const KParams &kp = s.kp;
-  typedef decltype(K::Smem::q[0][0][0]) RR;
-  typedef typename std::remove_const<typename std::remove_reference<RR>::type>::type R;
-  std::vector<LaneRegs<R>> regs(K::NT);
-  typename K::Smem *sm = (typename K::Smem *)malloc(sizeof(typename K::Smem));
-  const R nan = std::numeric_limits<R>::quiet_NaN();
-  for (int blk = 0; blk < s.g.nblocks; ++blk) {
-    int sx, sy;
-    strip_of_block(s.g, blk, sx, sy);
-    const int i0 = sx * STRIP_OUT, jb = sy * s.g.seg;
-    const int je = (jb + s.g.seg < kp.Ny) ? jb + s.g.seg : kp.Ny;
-    R *raw = (R *)sm;
-    for (size_t k = 0; k < sizeof(typename K::Smem) / sizeof(R); ++k) raw[k] = nan;
-    for (auto &r : regs) {
-      memset(&r, 0xff, sizeof r);  // garbage registers: every use must follow a definition
-      r.neg_rho = r.neg_prs = r.nan_cnt = 0;
-      r.inv = 0.0;
-    }
-#define LANES(stmt)                                                                                                    \
-  for (int t = 0; t < K::NT; ++t) {                                                                                    \
-    const int lane = t & 63, w = t >> 6;                                                                               \
-    LaneRegs<R> &r = regs[t];                                                                                          \
-    (void)lane;                                                                                                        \
-    (void)w;                                                                                                           \
-    (void)r;                                                                                                           \
-    stmt;                                                                                                              \
-  }
-    std::vector<RowPrefetch<S>> pfs(K::NT);
-    for (auto &p : pfs) memset(&p, 0xff, sizeof p);  // garbage until loaded
-    if (K::kPF && jb < je) LANES(K::chunk_prefetch(lane, w, 0, i0, jb, kp, io, pfs[t]));
-    LANES(K::pro_load(lane, w, i0, jb, *sm, kp, io));
-    LANES(K::pro_slopes(lane, w, jb, *sm));
-    LANES(K::pro_flux(lane, w, jb, *sm, kp));
-    int xw = 0;
-    for (int jc = jb; jc < je; jc += STRIP_ROWS) {
-      if (K::kPF) {
-        LANES(K::chunk_commit(lane, w, xw, i0, jb, jc, *sm, kp, io, pfs[t], r));
-        if (jc + STRIP_ROWS < je)
-          LANES(K::chunk_prefetch(lane, w, (xw + 1) & (STRIP_ROWS - 1), i0, jc + STRIP_ROWS, kp, io, pfs[t]));
-      } else {
-        LANES(K::chunk_load(lane, w, xw, i0, jb, jc, *sm, kp, io, r));
-      }
-      LANES(K::pass1_slopes(lane, w, jb, jc, *sm, r));
-      shift(regs, [](LaneRegs<R> &d, const LaneRegs<R> *s_) {
-        for (int v = 0; v < 4; ++v) d.sxl[v] = s_ ? s_->sx[v] : R(0);
-      });
-      LANES(K::pass1_flux(lane, w, jb, jc, *sm, kp, r));
-      LANES(K::pass2_flux(lane, w, jb, jc, *sm, kp, r));
-      LANES(K::update_faces(lane, w, jb, jc, *sm, kp, r));
-      shift(regs, [](LaneRegs<R> &d, const LaneRegs<R> *s_) {
-        for (int v = 0; v < 4; ++v) d.fxl[v] = s_ ? s_->fx[v] : R(0);
-        if (K::kTC) d.tcxl = s_ ? s_->tcx : R(0);
-        if (K::kVISC)
-          for (int v = 0; v < 3; ++v) d.vxl[v] = s_ ? s_->vx[v] : R(0);
-      });
-      LANES(K::update_cell(lane, w, i0, jb, jc, je, *sm, kp, io, s.ctl.dt, s.ctl.dt_dx, s.ctl.dt_dy, r));
-      xw = (xw + 1) & (STRIP_ROWS - 1);
-    }
-#undef LANES
-    if (io.epilogue) {
-      double m = 0.0;
-      for (auto &r : regs) {
-        m = fmax(m, r.inv);
-        s.ctl.neg_rho += r.neg_rho;
-        s.ctl.neg_prs += r.neg_prs;
-        s.ctl.nan_cnt += r.nan_cnt;
-      }
-      s.partials[blk] = m;
-    }
-  }
-  free(sm);
-}
-
-template <typename R, typename S, int RECON, int RIEM, bool XG>
-void stage_rrx(Sim &s, const StageIO<S> &io) {
-  const bool tc = s.p.tc_active, vi = s.p.visc_active;
-  if (tc && vi) run_stage<StripStage<R, S, RECON, RIEM, true, true, XG>, S>(s, io);
-  else if (tc) run_stage<StripStage<R, S, RECON, RIEM, true, false, XG>, S>(s, io);
-  else if (vi) run_stage<StripStage<R, S, RECON, RIEM, false, true, XG>, S>(s, io);
-  else run_stage<StripStage<R, S, RECON, RIEM, false, false, XG>, S>(s, io);
-}
-
-template <typename R, typename S, int RECON, int RIEM>
-void stage_rr(Sim &s, const StageIO<S> &io) {
-  const KParams &kp = s.kp;
-  if (kp.grav != GRAV_NONE || kp.wb || kp.bctc_ymin || kp.bctc_ymax) stage_rrx<R, S, RECON, RIEM, true>(s, io);
-  else stage_rrx<R, S, RECON, RIEM, false>(s, io);
-}
-
-template <typename R, typename S, int RECON>
-void stage_r(Sim &s, const StageIO<S> &io) {
-  switch (s.p.riemann_solver) {
-  case HLL: stage_rr<R, S, RECON, HLL>(s, io); break;
-  case FSLP: stage_rr<R, S, RECON, FSLP>(s, io); break;
-  default: stage_rr<R, S, RECON, HLLC>(s, io); break;
-  }
-}
 
 template <typename R, typename S>
 void stage(Sim &s, const S *src, int qsrc, const S *q0, const S *avg, S *out, int epilogue) {
@@ -160,8 +27,10 @@ void stage(Sim &s, const S *src, int qsrc, const S *q0, const S *avg, S *out, in
   io.partials = s.partials.data();
   io.qsrc = qsrc;
   io.epilogue = epilogue;
-  if (s.p.reconstruction == FV2D_PLM) stage_r<R, S, PLM>(s, io);
-  else stage_r<R, S, PCM>(s, io);
+  switch (s.g.nw) {
+  case 8: stage_nw<R, S, 8>(s, io); break;
+  default: stage_nw<R, S, 4>(s, io); break;
+  }
 }
 
 void reduce(Sim &s) {
@@ -267,13 +136,16 @@ void get_cons_t(const Sim &s, const std::vector<S> &U, double *u) {
 
 extern "C" {
 
-// Number of a[k] for which markstein_div(a, d, RN(1/d)) differs from a / d in any bit.
-long long emu_markstein_mismatches(const double *a, long long n, double d) {
+int emu_div_mode(double d) { return div_mode(d); }
+
+// Number of a[k] for which markstein_div(a, d, RN(1/d), mode) differs from a / d in any bit;
+// mode < 0: the mode the solver picks for d (div_mode).
+long long emu_markstein_mismatches(const double *a, long long n, double d, int mode) {
   const double rd = 1.0 / d;
-  const int fast = (d >= 0x1p-60 && d <= 0x1p+60) ? 1 : 0;
+  if (mode < 0) mode = div_mode(d);
   long long bad = 0;
   for (long long k = 0; k < n; ++k) {
-    const double x = markstein_div(a[k], d, rd, fast), y = a[k] / d;
+    const double x = markstein_div(a[k], d, rd, mode), y = a[k] / d;
     if (memcmp(&x, &y, sizeof x) != 0 && !(x != x && y != y)) ++bad;
   }
   return bad;
@@ -285,12 +157,16 @@ void *emu_create(const fv2d_params *p, char *err) {
     if (err) snprintf(err, 256, "%s", why.c_str());
     return nullptr;
   }
+  if (p->precision == FV2D_FP32_STATE64) {
+    if (err) snprintf(err, 256, "the FP32 form with an FP64 state is not emulated");
+    return nullptr;
+  }
   Sim *s = new Sim();
   s->p = *p;
   s->fp32 = p->precision == FV2D_FP32;
   const int pitch = (p->Nx + 63) / 64 * 64;
   build_kparams(p, pitch, s->kp);
-  s->g = strip_geom(p->Nx, p->Ny, pick_seg_rows(p));
+  s->g = sim_geom(p);
   s->partials.assign((size_t)s->g.nblocks, 0.0);
   const size_t n = 4 * (size_t)s->kp.plane;
   // unused padding columns hold NaN: a read of them would poison the results

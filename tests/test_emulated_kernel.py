@@ -70,6 +70,19 @@ def test_segment_heights_bitwise(Nx, Ny, seg, emu_lib, oracle_lib):
         compare(p.copy(seg_rows=seg), q0, emu_lib, oracle_lib, steps=2)
 
 
+@pytest.mark.parametrize("nw", [8])
+@pytest.mark.parametrize("Nx,Ny,seg", [(124, 66, 0), (125, 130, 12), (37, 19, 0), (130, 7, 4), (2, 2, 0),
+                                       (63, 45, 20)])
+def test_chunk_heights_bitwise(nw, Nx, Ny, seg, emu_lib, oracle_lib):
+    """Chunks of 8 rows (workgroups of 8 waves): default segments, segments
+    shorter than a chunk or not a multiple of it, grids shorter than one chunk."""
+    for n, c in enumerate([(1, 1, PLM, 1, 1, 1, 1, 1, 1), (2, 0, PLM, 0, 0, 0, 1, 0, 0), (0, 2, 0, 2, 1, 1, 0, 2, 1)]):
+        p, q0 = make(n, *c, Nx=Nx, Ny=Ny)
+        compare(p.copy(seg_rows=seg, chunk_rows=nw), q0, emu_lib, oracle_lib, steps=2)
+    if (Nx, Ny) == (124, 66):
+        fp32_close(emu_lib, oracle_lib, N=48, steps=6, chunk_rows=nw)
+
+
 def test_sanitized_bounds(oracle_lib):
     """ASan/UBSan build: every 9th combination and three strip / segment edge shapes, 2 steps;
     a bad index aborts the process, so this test runs the sanitized library in a child
@@ -86,9 +99,10 @@ def test_sanitized_bounds(oracle_lib):
         "    if n %% 9: continue\n"
         "    p, q0 = T.make(n, *c)\n"
         "    T.compare(p, q0, lib, olib, steps=2)\n"
-        "for Nx, Ny, seg in ((124, 66, 8), (130, 7, 4), (2, 2, 0)):\n"
+        "for Nx, Ny, seg, nw in ((124, 66, 8, 4), (130, 7, 4, 4), (2, 2, 0, 4), (124, 66, 0, 8), (63, 45, 20, 8),\n"
+        "                        (5, 3, 0, 8)):\n"
         "    p, q0 = T.make(0, 1, 1, 2, 1, 1, 1, 1, 1, 1, Nx=Nx, Ny=Ny)\n"
-        "    T.compare(p.copy(seg_rows=seg), q0, lib, olib, steps=2)\n"
+        "    T.compare(p.copy(seg_rows=seg, chunk_rows=nw), q0, lib, olib, steps=2)\n"
         "T.fp32_close(lib, olib, N=24, steps=4)\n"
         "print('ok')\n" % ([os.path.dirname(__file__), os.path.dirname(os.path.dirname(__file__)),
                            os.path.join(os.path.dirname(__file__), "emu")],))
@@ -105,10 +119,10 @@ def test_fp32_form_close_to_fp64(emu_lib, oracle_lib):
     fp32_close(emu_lib, oracle_lib)
 
 
-def fp32_close(emu_lib, oracle_lib, N=64, steps=20):
+def fp32_close(emu_lib, oracle_lib, N=64, steps=20, chunk_rows=0):
     p, q0, _ = P.workload("kh", N=N)
     o = Oracle(p, oracle_lib)
-    e = EMU.Emu(p.copy(precision=FP32), emu_lib)
+    e = EMU.Emu(p.copy(precision=FP32, chunk_rows=chunk_rows), emu_lib)
     o.set_prim(q0)
     e.set_prim(q0)
     for _ in range(steps):

@@ -355,3 +355,21 @@ def test_stream_size_fp64_job_equals_oracle(oracle_lib, gpu_lib):
     assert inf.nsteps == len(dts_o) == sp.steps
     assert inf.t == t_o, (inf.t, t_o)
     assert np.array_equal(q, qo), f"stream-size FP64 job differs from the oracle: rel Linf {rel_linf(q, qo):.3e}"
+
+
+def test_chunk_heights_in_one_batch(gpu_lib):
+    """Jobs with 4- and 8-row chunks (two kernel classes of one setup: the autotuner's
+    waves-per-workgroup candidates) in one batch, FP64 and FP32: each bitwise equal to the
+    same job alone with the default tile."""
+    for prec in (FP64, FP32):
+        specs = [spec(PROB_KH, 200, 64, 5 / 3, 7e-3, 5, steps=11), spec(PROB_KH, 130, 90, 1.45, 2e-3, 9, steps=7),
+                 spec(PROB_BLAST, 96, 96, 1.6, steps=9), spec(PROB_SOD_X, 256, 24, 1.55, steps=13)]
+        tiles = [(32, 8), (0, 4), (16, 8), (24, 8)]
+        jobs = [(S.job_params(sp, prec, seg, nw), S.job_problem(sp), None, sp.steps) for sp, (seg, nw) in
+                zip(specs, tiles)]
+        got = run_batch(jobs, prec, gpu_lib, stagger=2)
+        for k, sp in enumerate(specs):
+            qa, sa, ta, na = alone(sp, prec, gpu_lib)
+            q, sums, inf = got[k]
+            assert inf.nsteps == na == sp.steps and inf.t == ta, (prec, k)
+            assert np.array_equal(q, qa), f"{prec} job {k} tile {tiles[k]}: not bitwise equal alone"

@@ -99,6 +99,14 @@ def test_hydro_magnitudes(probe):
     check(run(probe, rng.normal(0.0, 1e-3, n), rng.normal(0.0, 1e-9, n), rng.normal(0.0, 1.0, n)))
 
 
+def div_mode(d):
+    """The mode the solver picks for a cell size d (div_mode in fv2d_math.h)."""
+    m, _ = np.frexp(d)
+    if 2.0 ** -1022 <= d <= 2.0 ** 1022 and m == 0.5:
+        return 2
+    return 1 if 2.0 ** -60 <= d <= 2.0 ** 60 else 0
+
+
 def test_markstein_on_device(probe):
     """The constant-divisor division (markstein_div) on the device, where lanes of one wave
     share the fast path or fall back together: cell sizes and random divisors, numerators
@@ -111,15 +119,17 @@ def test_markstein_on_device(probe):
                             np.tile(SPECIALS, 8)])
         rng.shuffle(a)
         a = np.ascontiguousarray(a)
-        out = np.zeros(2 * a.size)
         dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
-        fast = 1 if 2.0 ** -60 <= d <= 2.0 ** 60 else 0
-        assert probe.markstein_probe(dp(a), a.size, d, 1.0 / d, fast, dp(out)) == 0
-        out = out.reshape(-1, 2)
-        bad = ~same_bits(out[:, 0], out[:, 1])
-        assert not bad.any(), f"d={d!r}: {np.count_nonzero(bad)} mismatches, e.g. a={a[np.flatnonzero(bad)[:3]]}"
-        # whole waves of zeros and of ordinary values take the fast path
-        z = np.ascontiguousarray(np.concatenate([np.zeros(256), -np.zeros(256), rng.uniform(1, 2, 256)]))
-        out = np.zeros(2 * z.size)
-        assert probe.markstein_probe(dp(z), z.size, d, 1.0 / d, fast, dp(out)) == 0
-        assert same_bits(out[0::2], out[1::2]).all()
+        # the mode the solver picks (one multiply for a power of two) and the Markstein mode
+        for fast in sorted({div_mode(d), 1}):
+            out = np.zeros(2 * a.size)
+            assert probe.markstein_probe(dp(a), a.size, d, 1.0 / d, fast, dp(out)) == 0
+            out = out.reshape(-1, 2)
+            bad = ~same_bits(out[:, 0], out[:, 1])
+            assert not bad.any(), f"d={d!r} mode {fast}: {np.count_nonzero(bad)} mismatches, e.g. " \
+                                  f"a={a[np.flatnonzero(bad)[:3]]}"
+            # whole waves of zeros and of ordinary values take the fast path
+            z = np.ascontiguousarray(np.concatenate([np.zeros(256), -np.zeros(256), rng.uniform(1, 2, 256)]))
+            out = np.zeros(2 * z.size)
+            assert probe.markstein_probe(dp(z), z.size, d, 1.0 / d, fast, dp(out)) == 0
+            assert same_bits(out[0::2], out[1::2]).all()

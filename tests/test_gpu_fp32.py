@@ -1,4 +1,5 @@
-"""The FP32 form on the GPU against the FP64 form (which is bitwise the oracle's arithmetic)
+"""The FP32 form (state stored in FP32, and in FP64 for every switch combination) on the GPU
+against the FP64 form (which is bitwise the oracle's arithmetic)
 for every boundary type, reconstruction, Riemann solver and time stepping, with conduction,
 viscosity and constant gravity on, and with the gravity / wall-flux / wall-conduction
 variants: relative L1 of density and pressure <= 1e-5 (the stated bar) after 10 steps from
@@ -8,7 +9,7 @@ import itertools
 import numpy as np
 import pytest
 
-from fv2d_amd.capi import (BC_ABSORBING, BC_PERIODIC, BC_REFLECTING, FP32, FSLP, GRAV_ANALYTICAL, GRAV_CONSTANT, HLL,
+from fv2d_amd.capi import (BC_ABSORBING, BC_PERIODIC, BC_REFLECTING, FP32, FP32_STATE64, FSLP, GRAV_ANALYTICAL, GRAV_CONSTANT, HLL,
                            HLLC, PCM, PLM, Params)
 from fv2d_amd.solver import Solver
 
@@ -36,10 +37,10 @@ def run(p, q0):
         return s.get_prim(), dts, n
 
 
-def compare(p):
+def compare(p, form=FP32):
     q0 = smooth_state(p)
     q64, dts64, n64 = run(p, q0)
-    q32, dts32, n32 = run(p.copy(precision=FP32), q0)
+    q32, dts32, n32 = run(p.copy(precision=form), q0)
     assert n32 == n64 == STEPS
     assert np.all(np.abs(dts32 - dts64) <= 1e-5 * dts64), (dts32, dts64)
     for k, name in ((0, "rho"), (3, "prs")):
@@ -50,11 +51,12 @@ def compare(p):
 COMBOS = list(itertools.product([BC_ABSORBING, BC_REFLECTING, BC_PERIODIC], [PCM, PLM], [HLL, HLLC, FSLP], [0, 1]))
 
 
+@pytest.mark.parametrize("form", [FP32, FP32_STATE64])
 @pytest.mark.parametrize("bc,recon,riem,ts", COMBOS)
-def test_fp32_form_every_switch(bc, recon, riem, ts):
+def test_fp32_form_every_switch(bc, recon, riem, ts, form):
     compare(Params.defaults(Nx=67, Ny=41, boundary_x=bc, boundary_y=(bc + 1) % 3, reconstruction=recon,
                             riemann_solver=riem, time_stepping=ts, tc_active=1, kappa=3e-3, visc_active=1, mu=2e-3,
-                            gravity_mode=GRAV_CONSTANT, gx=0.3, gy=-0.7, gamma0=1.4, CFL=0.4))
+                            gravity_mode=GRAV_CONSTANT, gx=0.3, gy=-0.7, gamma0=1.4, CFL=0.4), form)
 
 
 @pytest.mark.parametrize("grav,wb,bctc", [(GRAV_CONSTANT, 1, 0), (GRAV_ANALYTICAL, 1, 1), (GRAV_CONSTANT, 0, 2)])

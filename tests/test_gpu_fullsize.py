@@ -1,6 +1,6 @@
 """The benchmark configurations at their full sizes on the GPU, checked through properties
-that do not depend on the grid size (the CPU oracle takes minutes at these sizes; the
-bitwise oracle comparison runs at reduced sizes in test_gpu_parity.py):
+that do not depend on the grid size (the comparison with the CPU oracle at these sizes is
+test_gpu_fullsize_oracle.py; at reduced sizes, every step, test_gpu_parity.py):
 
   * conservation to round-off: periodic KH 4096^2 (mass, both momenta, energy: the
     conduction and viscous fluxes are face fluxes too), Sedov 2048^2 with reflecting walls
@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from fv2d_amd import problems as P
-from fv2d_amd.capi import FP32
+from fv2d_amd.capi import FP32, FP32_STATE64
 from fv2d_amd.solver import Solver
 
 pytestmark = pytest.mark.gpu
@@ -70,10 +70,11 @@ def test_sod512_full_run_step_count():
     assert a["n"] == 406 and a["t"] >= 0.2 - 1e-6
 
 
+@pytest.mark.parametrize("prec", [FP32, FP32_STATE64])
 @pytest.mark.parametrize("name", ["sod", "sedov", "kh", "rt"])
-def test_fp32_form_error_every_config(name):
+def test_fp32_form_error_every_config(name, prec):
     a = run(name)
-    b = run(name, precision=FP32)
+    b = run(name, precision=prec)
     assert b["n"] == a["n"] or name == "sod", "same step count (fixed-step configurations)"
     errs = {}
     for k, f in enumerate(("rho", "u", "v", "prs")):
@@ -82,10 +83,11 @@ def test_fp32_form_error_every_config(name):
     assert errs["rho"][0] <= 1e-5 and errs["prs"][0] <= 1e-5, f"{name}: {errs}"
 
 
-@pytest.mark.parametrize("prec", [0, FP32])
+@pytest.mark.parametrize("prec", [0, FP32, FP32_STATE64])
 def test_kh4096_long_run(prec):
     """3000 steps (ten times the benchmark's): mass, momenta and energy conserved to round-off
-    over the whole run (relative 1e-11 in FP64; 1e-5 in the FP32 form, whose state is rounded
+    over the whole run (relative 1e-11 in FP64 and in the FP32 form with the state stored in
+    FP64, whose face fluxes still cancel exactly; 1e-5 in the FP32 form, whose state is rounded
     to FP32 when stored every stage: measured drift 1.1e-6 in mass), no NaN, and the run
     repeats bit for bit."""
     from fv2d_amd.capi import PROB_KH
@@ -100,7 +102,7 @@ def test_kh4096_long_run(prec):
 
     s0, s1, dts, counts, n, t = once()
     assert n == 3000 and counts[2] == 0 and np.all(dts > 0)
-    tol = 1e-11 if prec == 0 else 1e-5
+    tol = 1e-5 if prec == FP32 else 1e-11
     for k, name in enumerate(("mass", "x momentum", "y momentum", "energy")):
         scale = max(abs(s0[k]), abs(s0[0]))
         assert abs(s1[k] - s0[k]) <= tol * scale, f"{name}: {s0[k]!r} -> {s1[k]!r} after t = {t}"

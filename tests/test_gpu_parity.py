@@ -200,3 +200,26 @@ def test_hot_bubble_rows_on_gpu(oracle_lib, gpu_lib):
                         gamma0=1.4, CFL=0.5)
     q0 = P.random_state(p, seed=77)
     assert run_pair(p, q0, 6, oracle_lib, gpu_lib)
+
+
+@pytest.mark.parametrize("nw", [8])
+@pytest.mark.parametrize("Nx,Ny,seg", [(124, 66, 0), (125, 130, 12), (37, 19, 0), (130, 7, 4), (2, 2, 0),
+                                       (63, 45, 20)])
+def test_chunk_heights_bitwise(nw, Nx, Ny, seg, oracle_lib, gpu_lib):
+    """Workgroups of 8 waves (8-row chunks): default segments, segments
+    shorter than a chunk or not a multiple of it, grids shorter than one chunk."""
+    for n, (bcx, bcy, recon, riem, ts, tc, vi, grav, wb) in enumerate(SHAPE_SWITCHES):
+        p = Params.defaults(Nx=Nx, Ny=Ny, xmin=0.1, xmax=1.3, ymin=-0.2, ymax=0.7, boundary_x=bcx, boundary_y=bcy,
+                            reconstruction=recon, riemann_solver=riem, time_stepping=ts, tc_active=tc, kappa=3e-3,
+                            visc_active=vi, mu=2e-3, gravity_mode=grav, gx=0.3, gy=-0.7, hot_bubble_g0=0.5,
+                            well_balanced_flux_at_y_bc=wb, gamma0=1.4, CFL=0.4, seg_rows=seg, chunk_rows=nw)
+        q0 = P.random_state(p, seed=2000 * n + Nx + Ny + nw)
+        assert run_pair(p, q0, 3, oracle_lib, gpu_lib), f"{Nx}x{Ny} seg {seg} chunk {nw} switches {n}: not bitwise"
+
+
+@pytest.mark.parametrize("nw", [8])
+def test_chunk_heights_workloads(nw, oracle_lib, gpu_lib):
+    """The KH and RT workloads at reduced size with 8-row chunks, every fifth step."""
+    for name, size in (("kh", dict(N=128)), ("rt", dict(Nx=64, Ny=192))):
+        p, q0, _ = P.workload(name, **size)
+        assert run_pair(p.copy(chunk_rows=nw), q0, 10, oracle_lib, gpu_lib, every=5)
