@@ -11,7 +11,7 @@ synthetic seeded initial state, all state resident in HBM. One step = one full R
 (Gcell-updates/s = Nx*Ny*K / time of K steps, HIP events on the solver stream, device
 synchronised on both sides); the FP32 form is timed the same way and reported beside it.
 The K steps run as one device loop (fv2d_advance); `repetitions` gives the median, minimum
-and maximum rate of --reps further loops of K steps each (when K >= 32). `pcie_inclusive`
+and maximum rate of --reps further loops of K steps each (when K >= 16). `pcie_inclusive`
 adds the upload of the host primitives before and their read-back after (host clock).
 
 roofline: the dominant kernel is the fused stage kernel (two launches per step). Its
@@ -33,8 +33,11 @@ FP32 form's error against it. A seeded 10% subsample also runs one job at a time
 reference's loop structure (dt read back to the host, one blocking step per RK2 step).
 
 cpu_baseline: the CPU oracle (a plain-C FP64 restatement of the reference path with the
-reference's pass structure, OpenMP) timed on the host cores on a bounded sample of the
-same workload (as many steps as fit in --cpu-seconds, at least one).
+reference's pass structure, OpenMP) on every host core it may use (OMP_NUM_THREADS, else
+nproc), median of 3 bounded samples of the same workload (as many steps as fit in
+--cpu-seconds each), with nproc, the CPU model, OMP_NUM_THREADS and the host's STREAM-triad
+bandwidth. The FP32 form's error at every configuration is measured against the oracle's
+fields at full size (the oracle runs on the host cores; a few minutes).
 """
 from __future__ import annotations
 
@@ -53,7 +56,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s & % HBM peak at 4096² FP64 vs FP32; served-job p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_STEP = {"fp64": 160.0, "fp32": 80.0}
+BYTES_PER_CELL_STEP = {"fp64": 160.0, "fp32": 80.0, "fp32s64": 160.0}
 STAGE_LAUNCHES_PER_STEP = 2
 
 
@@ -73,10 +76,10 @@ def read_traffic(prec: str, size: int):
 def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps: int = 5):
     from fv2d_amd import hiprt
     from fv2d_amd import problems as P
-    from fv2d_amd.capi import FP32, FP64
+    from fv2d_amd.capi import FP32, FP32_STATE64, FP64
     from fv2d_amd.solver import Solver
 
-    p = P.kh_params(N=size, precision=FP64 if prec == "fp64" else FP32)
+    p = P.kh_params(N=size, precision={"fp64": FP64, "fp32": FP32, "fp32s64": FP32_STATE64}[prec])
     s = Solver(p)
     hiprt.synchronize()
     t_up = time.perf_counter()
@@ -104,9 +107,9 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps:
     assert n == steps, f"advance took {n} steps, expected {steps}"
     cells = p.Nx * p.Ny
     # repetitions (spread): `reps` more device loops of the same K steps, each event-timed;
-    # only when a loop is long enough (>= 32 steps) for its rate to mean something
+    # only when a loop is long enough (>= 16 steps) for its rate to mean something
     rep_rates = []
-    if steps >= 32:
+    if steps >= 16:
         for _ in range(reps):
             ev[0].record(stream)
             t, _, _, nr = s.advance(max_steps=steps, t0=t, record_dts=False)
@@ -133,26 +136,55 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps:
                 blocks=reps_out)
 
 
-def run_cpu(size: int, seconds: float, q0: np.ndarray):
+def host_info():
+    """nproc, the CPU model and OMP_NUM_THREADS of the host the baseline runs on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return {"nproc": os.cpu_count(), "cpu_model": model, "OMP_NUM_THREADS": omp}
+
+
+def run_cpu(size: int, seconds: float, q0: np.ndarray, samples: int = 3):
+    """The CPU baseline: the oracle (the reference path restated in C, its pass structure,
+    OpenMP) on every host core it may use (OMP_NUM_THREADS when set, else nproc), on a
+    bounded sample of the KH workload: `samples` runs of as many RK2 steps as fit in
+    `seconds` each (at least one step), median reported; plus the host's STREAM-triad
+    bandwidth in the same process for context."""
     from fv2d_amd import problems as P
     from oracle import oracle as OR
 
-    threads = min(os.cpu_count() or 1, 16)
+    info = host_info()
+    threads = int(info["OMP_NUM_THREADS"]) if (info["OMP_NUM_THREADS"] or "").isdigit() else (os.cpu_count() or 1)
     lib = OR.load()
     lib.orc_set_threads(threads)
+    triad = lib.orc_stream_triad(1 << 26, 5)  # 3 x 512 MiB arrays
+    rates, steps = [], []
     o = OR.Oracle(P.kh_params(N=size), lib)
     o.set_prim(q0)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        dt, _ = o.compute_dt()
-        o.step(dt)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 50:
-            break
+    dt, _ = o.compute_dt()
+    o.step(dt)  # warm-up: first touch of every array
+    for _ in range(samples):
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            dt, _ = o.compute_dt()
+            o.step(dt)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or n >= 200:
+                break
+        rates.append(size * size * n / el / 1e9)
+        steps.append(n)
     o.close()
-    return dict(value=size * size * n / el / 1e9, steps=n, seconds=el, threads=lib.orc_get_threads())
+    return dict(value=float(np.median(rates)), samples=[round(r, 5) for r in rates], steps=steps,
+                seconds=seconds, threads=lib.orc_get_threads(), stream_triad_gbs=round(triad, 1), **info)
 
 
 def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: int = 2024, use_graph: bool = True):
@@ -163,7 +195,7 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
     from fv2d_amd.capi import FP32, FP64, BatchConfig
 
     t0 = time.perf_counter()
-    tune = S.autotune(FP32, segs=(32, 64), blocks_per_cu=(0,), log=log, use_graph=int(use_graph))
+    tune = S.autotune(FP32, log=log, use_graph=int(use_graph))
     t_tune = time.perf_counter() - t0
     specs = S.make_stream(n_jobs, rate=0, seed=seed, steps=steps)
     rng = np.random.default_rng(seed + 1)
@@ -250,36 +282,83 @@ def run_serving(n_jobs: int, steps: int, load: float, check_frac: float, seed: i
                                              "under the same arrivals"},
         "latency_all_queued_ms": {k: round(v * 1e3, 2) for k, v in rb.latency_percentiles().items()},
         "fp64_check": fp64_check,
-        "autotune": {"seconds": round(t_tune, 1), "seg_choice": tune.get("seg_choice"), "seg": tune["seg"]},
+        "autotune": {"seconds": round(t_tune, 1), "tile_choice": tune.get("tile_choice"),
+                     "blocks_per_cu": tune.get("blocks_per_cu") or "occupancy",
+                     "winner_per_class": {k: {"seg_rows": v[0], "chunk_rows": v[1]} for k, v in tune["tile"].items()},
+                     "candidates_gcells": {k: round(v / 1e9, 3) for k, v in tune["rates"].items()}},
     }
 
 
-def fp32_errors_all_configs():
-    """FP32 form against the FP64 form (bitwise the oracle's arithmetic) at every benchmark
-    configuration, full size: relative L1 and Linf error per primitive field."""
+def errors_vs_oracle_all_configs():
+    """Every benchmark configuration at full size against the CPU oracle (FP64, the
+    reference path restated in C, on the host cores): the FP64 form's parity (same step
+    count and dt sequence, worst relative L-infinity, bitwise equality) and the FP32 form's
+    relative L1 and L-infinity error per primitive field."""
     from fv2d_amd import problems as P
-    from fv2d_amd.capi import FP32, FP64
+    from fv2d_amd.capi import FP32, FP32_STATE64, FP64
+    from fv2d_amd.solver import Solver
+    from oracle import oracle as OR
+
+    lib = OR.load()
+    info = host_info()
+    lib.orc_set_threads(int(info["OMP_NUM_THREADS"]) if (info["OMP_NUM_THREADS"] or "").isdigit()
+                        else (os.cpu_count() or 1))
+    out = {}
+    for name in ("sod", "sedov", "kh", "rt"):
+        p, q0, r = P.workload(name)
+        steps = r["steps"] or 0
+        t0 = time.perf_counter()
+        with OR.Oracle(p, lib) as o:
+            o.set_prim(q0)
+            _, dts_o, _ = o.run(r["tend"], steps)
+            qo = o.get_prim()
+        t_oracle = time.perf_counter() - t0
+        res = {}
+        for prec in (FP64, FP32, FP32_STATE64):
+            with Solver(p.copy(precision=prec)) as s:
+                s.set_prim(q0)
+                _, dts, _, n = s.advance(tend=r["tend"], max_steps=steps or 100000)
+                res[prec] = (s.get_prim(), n, dts)
+        def field_err(q):
+            e = {}
+            for k, f in enumerate(("rho", "u", "v", "prs")):
+                d = np.abs(q[k] - qo[k])
+                e[f] = {"L1": float(d.mean() / max(np.abs(qo[k]).mean(), 1e-300)),
+                        "Linf": float(d.max() / max(np.abs(qo[k]).max(), 1e-300))}
+            return e
+
+        e = field_err(res[FP32][0])
+        e64s = field_err(res[FP32_STATE64][0])
+        q64 = res[FP64][0]
+        linf64 = max(float(np.abs(q64[k] - qo[k]).max() / max(np.abs(qo[k]).max(), 1e-300)) for k in range(4))
+        out[name] = {"grid": [p.Nx, p.Ny], "steps_oracle": len(dts_o), "steps_fp64": res[FP64][1],
+                     "steps_fp32": res[FP32][1], "oracle_seconds": round(t_oracle, 1),
+                     "fp64_parity": {"same_dt_sequence": bool(np.array_equal(res[FP64][2], dts_o)),
+                                     "rel_Linf": linf64, "bitwise": bool(np.array_equal(q64, qo))},
+                     "fp32_rel_error": e, "fp32_state64_steps": res[FP32_STATE64][1],
+                     "fp32_state64_rel_error": e64s}
+        log(f"bench: {name} vs oracle ({t_oracle:.0f} s): FP64 rel Linf {linf64:.1e}, "
+            f"FP32 L1(rho) {e['rho']['L1']:.2e}")
+    return out
+
+
+def kh_long_run_drift(size: int = 4096, steps: int = 3000):
+    """Relative drift of the conserved totals (mass, x / y momentum, energy; momenta relative
+    to the mass) over `steps` KH steps, for the FP64 form and both FP32 forms."""
+    from fv2d_amd.capi import FP32, FP32_STATE64, FP64, PROB_KH
+    from fv2d_amd import problems as P
     from fv2d_amd.solver import Solver
 
     out = {}
-    for name in ("sod", "sedov", "kh", "rt"):
-        res = {}
-        for prec in (FP64, FP32):
-            p, q0, r = P.workload(name)
-            with Solver(p.copy(precision=prec)) as s:
-                s.set_prim(q0)
-                _, _, _, n = s.advance(tend=r["tend"], max_steps=r["steps"] or 0, record_dts=False)
-                res[prec] = (s.get_prim(), n)
-        a, b = res[FP64][0], res[FP32][0]
-        e = {}
-        for k, f in enumerate(("rho", "u", "v", "prs")):
-            d = np.abs(b[k] - a[k])
-            e[f] = {"L1": float(d.mean() / max(np.abs(a[k]).mean(), 1e-300)),
-                    "Linf": float(d.max() / max(np.abs(a[k]).max(), 1e-300))}
-        p, _, _ = P.workload(name)
-        out[name] = {"grid": [p.Nx, p.Ny], "steps_fp64": res[FP64][1], "steps_fp32": res[FP32][1], "rel_error": e}
-        log(f"bench: FP32 vs FP64 {name}: L1(rho) {e['rho']['L1']:.2e}")
-    return out
+    for name, prec in (("fp64", FP64), ("fp32", FP32), ("fp32s64", FP32_STATE64)):
+        with Solver(P.kh_params(N=size, precision=prec)) as s:
+            s.set_problem(PROB_KH, seed=12345, amplitude=0.01)
+            s0 = s.sums()
+            s.advance(max_steps=steps, record_dts=False)
+            s1 = s.sums()
+        out[name] = {k: float(abs(s1[i] - s0[i]) / max(abs(s0[i]), abs(s0[0])))
+                     for i, k in enumerate(("mass", "x_momentum", "y_momentum", "energy"))}
+    return {"steps": steps, "grid": [size, size], "rel_drift": out}
 
 
 def calibrate_copy(mib: int, reps: int = 4) -> int:
@@ -315,8 +394,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--reps", type=int, default=5,
-                    help="further event-timed device loops of K steps for the spread (median reported; K >= 32)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+                    help="further event-timed device loops of K steps for the spread (median reported; K >= 16)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="length of each of the 3 CPU-baseline samples")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -364,17 +443,23 @@ def main(argv=None):
         log(f"bench: FP32 form, {a.warmup} warmup + {a.steps} timed steps")
         r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0, a.reps)
         log(f"bench: FP32 {r32['gcells']:.3f} Gcell/s, {r32['ms_per_step']:.3f} ms/step")
-    fp32_err = None
+        log(f"bench: FP32 form with the state in FP64, {a.warmup} warmup + {a.steps} timed steps")
+        r32s = run_gpu("fp32s64", a.size, a.steps, a.warmup, q0, a.reps)
+        log(f"bench: FP32 (state FP64) {r32s['gcells']:.3f} Gcell/s, {r32s['ms_per_step']:.3f} ms/step")
+    fp32_err = drift = None
     if not a.no_fp32 and not a.no_errors and a.size == 4096:
-        fp32_err = fp32_errors_all_configs()
+        drift = kh_long_run_drift(a.size)
+        log(f"bench: KH 3000-step mass drift: {drift['rel_drift']}")
+        fp32_err = errors_vs_oracle_all_configs()
     serving = None
     if not a.no_serve:
         serving = run_serving(a.serve_jobs, a.serve_steps, a.serve_load, 0.05)
     cpu = None
     if not a.no_cpu:
-        log(f"bench: CPU oracle baseline, ~{a.cpu_seconds:.0f} s sample")
+        log(f"bench: CPU oracle baseline, 3 x ~{a.cpu_seconds:.0f} s samples")
         cpu = run_cpu(a.size, a.cpu_seconds, q0)
-        log(f"bench: CPU {cpu['value']:.4f} Gcell/s over {cpu['steps']} steps on {cpu['threads']} threads")
+        log(f"bench: CPU {cpu['value']:.4f} Gcell/s (median of {cpu['samples']}) on {cpu['threads']} threads, "
+            f"STREAM triad {cpu['stream_triad_gbs']} GB/s")
 
     cells = a.size * a.size
     out = {
@@ -441,8 +526,20 @@ def main(argv=None):
             },
             "rel_L1_vs_fp64_after_steps": l1,
         }
+        out["fp32_form"]["state_fp64"] = {
+            "value": round(r32s["gcells"], 4),
+            "ms_per_step": round(r32s["ms_per_step"], 4),
+            "speedup_vs_fp64": round(r32s["gcells"] / r64["gcells"], 3),
+            "roofline": {"bound": "hbm", "achieved": round(r32s["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(r32s["achieved"] / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_launch": r32s["bytes_per_launch"],
+                         "mean_launch_ms": round(r32s["mean_launch_ms"], 4)},
+            "note": "the FP32 form's arithmetic with the state U stored in FP64 (160 B per cell-step)",
+        }
+        if drift is not None:
+            out["fp32_form"]["kh_long_run"] = drift
         if fp32_err is not None:
-            out["fp32_form"]["error_vs_fp64_every_config"] = fp32_err
+            out["fp32_form"]["error_vs_oracle_every_config"] = fp32_err
     if serving is not None:
         out["serving"] = serving
     if cpu is not None:
@@ -451,8 +548,14 @@ def main(argv=None):
             "unit": "Gcell-updates/s",
             "cores": cpu["threads"],
             "kind": "port",
-            "sample": f"CPU oracle (FP64 restatement of the reference path, OpenMP), KH {a.size}^2, "
-                      f"{cpu['steps']} RK2 steps in {cpu['seconds']:.1f} s",
+            "sample": f"CPU oracle (FP64 restatement of the reference path with its pass structure, OpenMP), "
+                      f"KH {a.size}^2: median of {len(cpu['samples'])} runs of ~{cpu['seconds']:.0f} s "
+                      f"({cpu['steps']} RK2 steps)",
+            "samples": cpu["samples"],
+            "nproc": cpu["nproc"],
+            "cpu_model": cpu["cpu_model"],
+            "OMP_NUM_THREADS": cpu["OMP_NUM_THREADS"],
+            "stream_triad_gbs": cpu["stream_triad_gbs"],
         }
     print(json.dumps(out), flush=True)
     return 0

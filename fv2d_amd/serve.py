@@ -55,10 +55,12 @@ class JobSpec:
         return self.Nx * self.Ny
 
 
-def job_params(spec: JobSpec, precision: int = FP32, seg_rows: int = 0) -> Params:
-    """Solver settings of a served job: the numerics of its setup on [0,1]^2."""
+def job_params(spec: JobSpec, precision: int = FP32, seg_rows: int = 0, chunk_rows: int = 0) -> Params:
+    """Solver settings of a served job: the numerics of its setup on [0,1]^2; seg_rows and
+    chunk_rows (the tile: rows per work item, rows per chunk = waves per workgroup) as tuned."""
     base = dict(Nx=spec.Nx, Ny=spec.Ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, gamma0=spec.gamma,
-                reconstruction=PLM, time_stepping=TS_RK2, precision=precision, seg_rows=seg_rows)
+                reconstruction=PLM, time_stepping=TS_RK2, precision=precision, seg_rows=seg_rows,
+                chunk_rows=chunk_rows)
     if spec.kind == PROB_SOD_X:
         base.update(riemann_solver=HLLC, CFL=0.8, boundary_x=BC_ABSORBING, boundary_y=BC_PERIODIC)
     elif spec.kind == PROB_BLAST:
@@ -136,10 +138,12 @@ class ServeResult:
         return {f"p{p}": float(np.percentile(lat, p)) for p in ps} if len(lat) else {}
 
 
-def _tuned_seg(tuning: dict | None, spec: JobSpec) -> int:
+def _tuned_tile(tuning: dict | None, spec: JobSpec) -> tuple[int, int]:
+    """(seg_rows, chunk_rows) the autotuner chose for the job's setup and grid (0, 0: defaults)."""
     if not tuning:
-        return 0
-    return int(tuning.get("seg", {}).get(f"{KIND_NAMES[spec.kind]}:{spec.Nx}x{spec.Ny}", 0))
+        return 0, 0
+    t = tuning.get("tile", {}).get(f"{KIND_NAMES[spec.kind]}:{spec.Nx}x{spec.Ny}", (0, 0))
+    return int(t[0]), int(t[1])
 
 
 def serve_batched(specs: list[JobSpec], precision: int = FP32, config: BatchConfig | None = None,
@@ -171,7 +175,7 @@ def serve_batched(specs: list[JobSpec], precision: int = FP32, config: BatchConf
             while queue:
                 s = queue[0]
                 try:
-                    jid = b.submit(job_params(s, precision, _tuned_seg(tuning, s)), problem=job_problem(s),
+                    jid = b.submit(job_params(s, precision, *_tuned_tile(tuning, s)), problem=job_problem(s),
                                    max_steps=s.steps)
                 except BatchBusy as e:
                     if not running and not to_collect:
@@ -264,7 +268,9 @@ def fifo_latencies(specs: list[JobSpec], service: dict) -> np.ndarray:
 
 # ------------------------------------------------------------------ autotuner
 
-SEG_CANDIDATES = (16, 32, 64, 128)
+# Tile candidates of the batched stage kernel: (segment height = rows per work item, chunk
+# rows = waves per workgroup).
+TILE_CANDIDATES = ((32, 4), (64, 4), (32, 8), (64, 8))
 
 
 def _time_batch(specs_params, config: BatchConfig, launches: int = 3, warm: int = 1, lib=None) -> float:
@@ -284,13 +290,15 @@ def _time_batch(specs_params, config: BatchConfig, launches: int = 3, warm: int 
 
 
 def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells: int = 12 << 20,
-             segs=SEG_CANDIDATES, blocks_per_cu=(0, 3, 2), mix_segs=(64,), cache: str | None = None,
+             tiles=TILE_CANDIDATES, blocks_per_cu=(0, 3, 2), mix_tiles=((64, 4), (64, 8)), cache: str | None = None,
              lib=None, log=None, use_graph: int = 1) -> dict:
-    """Choose the segment height (rows per work item: the tile height of the stage kernel)
-    per (setup, grid) and the persistent workgroups per CU of the batch: first by timing
-    batches of identical jobs (enough copies to fill the GPU) for each candidate height,
-    then on a mixed sample of the stream, where the per-size table competes with uniform
-    heights (`mix_segs`) and the workgroup counts. Results are cached in `cache` (JSON)."""
+    """Choose the tile of the batched stage kernel per kernel class (setup) and grid: the
+    segment height (rows per work item) and the chunk height (rows per chunk = waves per
+    workgroup, a compiled variant), then the persistent workgroups per CU of the batch.
+    First by timing batches of identical jobs (enough copies to fill the GPU) for each
+    candidate tile; then on a mixed sample of the stream, where the per-size table competes
+    with uniform tiles (`mix_tiles`), then the workgroup counts. Every candidate's rate is
+    kept in "rates", the choice per class and grid in "tile". Cached in `cache` (JSON)."""
     lib = lib or capi.load_library()
     key = f"prec{precision}"
     if cache and os.path.exists(cache):
@@ -298,55 +306,56 @@ def autotune(precision: int = FP32, sizes=SERVE_SIZES, kinds=KINDS, target_cells
             data = json.load(f)
         if key in data:
             return data[key]
-    out = {"seg": {}, "rates": {}}
+    out = {"tile": {}, "rates": {}}
     for kind in kinds:
         for nx, ny in sizes:
             spec = JobSpec(0, 0.0, kind, nx, ny, 1.4, 5e-3, 1, 0)
             copies = max(1, min(256, target_cells // (nx * ny)))
-            best, best_rate = 0, -1.0
-            for seg in segs:
-                if seg > (ny + 3) // 4 * 4 and seg != segs[0]:
+            best, best_rate = tiles[0], -1.0
+            for seg, nw in tiles:
+                if seg > (ny + nw - 1) // nw * nw and (seg, nw) != tiles[0]:
                     continue
-                p = job_params(spec, precision, seg)
+                p = job_params(spec, precision, seg, nw)
                 cfg = BatchConfig.defaults(precision=precision, max_jobs=max(copies, 1), use_graph=use_graph)
                 rate = _time_batch([(p, job_problem(spec))] * copies, cfg, lib=lib)
-                out["rates"][f"{KIND_NAMES[kind]}:{nx}x{ny}:seg{seg}"] = rate
+                out["rates"][f"{KIND_NAMES[kind]}:{nx}x{ny}:seg{seg}:nw{nw}"] = rate
                 if rate > best_rate:
-                    best, best_rate = seg, rate
-            out["seg"][f"{KIND_NAMES[kind]}:{nx}x{ny}"] = best
+                    best, best_rate = (seg, nw), rate
+            out["tile"][f"{KIND_NAMES[kind]}:{nx}x{ny}"] = list(best)
             if log:
-                log(f"autotune {KIND_NAMES[kind]} {nx}x{ny}: seg {best} ({best_rate / 1e9:.2f} Gcell/s)")
+                log(f"autotune {KIND_NAMES[kind]} {nx}x{ny}: seg {best[0]} x {best[1]} waves "
+                    f"({best_rate / 1e9:.2f} Gcell/s)")
     # On a mixed sample of the stream the classes run concurrently, which favours other
-    # segment heights than batches of one class: the per-size table and uniform heights
-    # compete there, then the persistent workgroups per CU. Heights above 64 rows were
-    # measured to add ~2% throughput but ~50% p50 latency (every job of a class waits for
-    # the slowest work item of each step), so they are not candidates by default.
+    # tiles than batches of one class: the per-size table and uniform tiles compete there,
+    # then the persistent workgroups per CU. Heights above 64 rows were measured to add ~2%
+    # throughput but ~50% p50 latency (every job of a class waits for the slowest work item
+    # of each step), so they are not candidates by default.
     sample = make_stream(96, rate=0, seed=99, steps=0, sizes=sizes)
 
     def mix_rate(table, bpc):
-        mix = [(job_params(s, precision, int(table.get(f"{KIND_NAMES[s.kind]}:{s.Nx}x{s.Ny}", 0))), job_problem(s))
-               for s in sample]
+        mix = [(job_params(s, precision, *table.get(f"{KIND_NAMES[s.kind]}:{s.Nx}x{s.Ny}", (0, 0))),
+                job_problem(s)) for s in sample]
         cfg = BatchConfig.defaults(precision=precision, max_jobs=len(mix), blocks_per_cu=bpc, use_graph=use_graph)
         return _time_batch(mix, cfg, lib=lib)
 
-    tables = {"per-size": dict(out["seg"])}
-    for seg in mix_segs:
-        tables[f"uniform-{seg}"] = {k: seg for k in out["seg"]}
+    tables = {"per-size": dict(out["tile"])}
+    for seg, nw in mix_tiles:
+        tables[f"uniform-seg{seg}-nw{nw}"] = {k: [seg, nw] for k in out["tile"]}
     best_t, best_rate = "per-size", -1.0
     for name, table in tables.items():
         rate = mix_rate(table, 0)
         out["rates"][f"mix:{name}"] = rate
         if rate > best_rate:
             best_t, best_rate = name, rate
-    out["seg"] = tables[best_t]
-    out["seg_choice"] = best_t
+    out["tile"] = tables[best_t]
+    out["tile_choice"] = best_t
     if log:
-        log(f"autotune mix: segment table {best_t} ({best_rate / 1e9:.2f} Gcell/s)")
+        log(f"autotune mix: tile table {best_t} ({best_rate / 1e9:.2f} Gcell/s)")
     best_b = 0
     for bpc in blocks_per_cu:
         if bpc == 0:
             continue
-        rate = mix_rate(out["seg"], bpc)
+        rate = mix_rate(out["tile"], bpc)
         out["rates"][f"mix:bpc{bpc}"] = rate
         if rate > best_rate:
             best_b, best_rate = bpc, rate

@@ -949,3 +949,47 @@ void orc_fslp(const double qL[4], const double qR[4], double gamma0, double fslp
   fslp(load4(qL), load4(qR), &f, pout, gdx, gamma0, fslp_K);
   store4(flux, f);
 }
+
+/* ---------------------------------------------------------------- host bandwidth */
+
+/* STREAM triad a[i] = b[i] + s * c[i] over n doubles per array with orc_get_threads()
+ * threads (first touch by the same static schedule), best of reps passes, in GB/s at the
+ * STREAM convention of 24 bytes per element. Context for the CPU baseline: the restated
+ * reference path is bound by host memory bandwidth. */
+double orc_stream_triad(long n, int reps) {
+  double *a = (double *)malloc(sizeof(double) * (size_t)n);
+  double *b = (double *)malloc(sizeof(double) * (size_t)n);
+  double *c = (double *)malloc(sizeof(double) * (size_t)n);
+  if (!a || !b || !c) {
+    free(a);
+    free(b);
+    free(c);
+    return 0.0;
+  }
+  OMP_FOR
+  for (long i = 0; i < n; ++i) {
+    a[i] = 0.0;
+    b[i] = 1.0;
+    c[i] = 2.0;
+  }
+  double best = 0.0;
+  const double s = 3.0;
+  for (int r = 0; r < reps; ++r) {
+#ifdef _OPENMP
+    const double t0 = omp_get_wtime();
+#endif
+    OMP_FOR
+    for (long i = 0; i < n; ++i) a[i] = b[i] + s * c[i];
+#ifdef _OPENMP
+    const double el = omp_get_wtime() - t0;
+    const double gbs = 24.0 * (double)n / el / 1e9;
+    if (gbs > best) best = gbs;
+#endif
+  }
+  volatile double sink = a[n / 2];
+  (void)sink;
+  free(a);
+  free(b);
+  free(c);
+  return best;
+}

@@ -69,6 +69,9 @@ double orc_minmod(double dL, double dR);
 /* getGravity (Gravity.h:39-57) at cell (i, j) for direction dir (0 = x, 1 = y). */
 double orc_gravity(const orc_sim *s, int i, int j, int dir);
 
+/* Host memory bandwidth: best STREAM-triad GB/s over reps passes of n doubles per array. */
+double orc_stream_triad(long n, int reps);
+
 /* Mesh facts. */
 void orc_mesh(const orc_sim *s, int32_t out_int[8], double out_real[2]);
 

@@ -58,6 +58,7 @@ def _declare(lib):
         "orc_minmod": (C.c_double, [C.c_double, C.c_double]),
         "orc_gravity": (C.c_double, [H, C.c_int, C.c_int, C.c_int]),
         "orc_mesh": (None, [H, P(C.c_int32), dp]),
+        "orc_stream_triad": (C.c_double, [C.c_long, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
