@@ -59,23 +59,13 @@ __device__ inline void commit_epilogue(unsigned nr, unsigned np, unsigned nn, do
 #define FV2D_WPE_BATCH(K) K::kWPEBatch
 #endif
 
-// One workgroup's share of a stage: strip/segment `blk` of the simulation described by kp,
-// io, g (prologue, then the chunk loop; see fv2d_stage.h). Shared by the single-simulation
-// kernel and the batched (persistent, work-list) kernel.
+// Rows [jb, je) of the strip starting at column i0 - 2: the segment prologue, then the chunk
+// loop.
 template <class K, typename R, typename S>
-__device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, const StripGeom &g, int blk, double dt,
-                                   double dt_dx, double dt_dy, typename K::Smem &sm) {
+__device__ inline void strip_rows(const KParams &kp, const StageIO<S> &io, int i0, int jb, int je, double dt,
+                                  double dt_dx, double dt_dy, typename K::Smem &sm, LaneRegs<R> &r) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: ring slots in SGPRs
-  int sx, sy;
-  strip_of_block(g, blk, sx, sy);
-  const int i0 = sx * STRIP_OUT, jb = sy * g.seg;
-  const int je = (jb + g.seg < kp.Ny) ? jb + g.seg : kp.Ny;
-
-  LaneRegs<R> r;
-  r.neg_rho = r.neg_prs = r.nan_cnt = 0;
-  r.inv = 0.0;
-
   RowPrefetch<S> pf;
   if (K::kPF && jb < je) K::chunk_prefetch(lane, w, 0, i0, jb, kp, io, pf);
   K::pro_load(lane, w, i0, jb, sm, kp, io);
@@ -116,6 +106,23 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
     __syncthreads();
     xw = (xw + 1) % K::NW;
   }
+}
+
+// One workgroup's share of a stage: block `blk` of the simulation described by kp, io, g
+// (its strip and rows from block_rows; a block without rows only reports a zero dt partial).
+// Shared by the single-simulation kernel and the batched (persistent, work-list) kernel.
+template <class K, typename R, typename S>
+__device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, const StripGeom &g, int blk, double dt,
+                                   double dt_dx, double dt_dy, typename K::Smem &sm) {
+  int sx, jb, je;
+  block_rows(g, kp.Ny, blk, sx, jb, je);
+  const int i0 = sx * STRIP_OUT;
+
+  LaneRegs<R> r;
+  r.neg_rho = r.neg_prs = r.nan_cnt = 0;
+  r.inv = 0.0;
+
+  if (jb < je) strip_rows<K, R, S>(kp, io, i0, jb, je, dt, dt_dx, dt_dy, sm, r);
   if (io.epilogue) commit_epilogue<K::NT>(r.neg_rho, r.neg_prs, r.nan_cnt, r.inv, io.ctl, io.partials, blk);
 }
 
@@ -185,6 +192,7 @@ batch_stage_kernel(const BatchJob *__restrict__ jobs, const BatchItem *__restric
     g.seg = J.seg;
     g.nblocks = J.nblocks;
     g.nw = K::NW;
+    g.sched = 0;  // uniform segments: the work list holds one item per (strip, segment)
     strip_block<K, R, S>(J.kp, io, g, wi.blk, uni(ctl->dt), uni(ctl->dt_dx), uni(ctl->dt_dy), sm);
     __syncthreads();
   }
@@ -250,8 +258,18 @@ auto with_class(const StageClass &c, F &&f) {
 
 template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW>
 hipError_t launch_stage_n(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
-  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW> K;
-  hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
+  if constexpr (sizeof(R) == 8) {
+    if (kp.divx == DIV_EXACT && kp.divy == DIV_EXACT) {
+      typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW, DIVF_EXACT> K;
+      hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
+    } else {
+      typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW, DIVF_GENERAL> K;
+      hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
+    }
+  } else {
+    typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW> K;
+    hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
+  }
   return hipGetLastError();
 }
 

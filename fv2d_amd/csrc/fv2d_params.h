@@ -134,10 +134,14 @@ inline int pick_seg_rows(const fv2d_params *p, int nw) {
   return seg;
 }
 
-// Strip geometry of the single-grid kernel.
+// Strip geometry of the single-grid kernel: the banded long-then-short schedule when the
+// uniform segments make at least 2048 workgroups (two rounds of 4 per CU on 256 CUs), else
+// the uniform one.
 inline StripGeom sim_geom(const fv2d_params *p) {
   const int nw = pick_chunk_rows(p);
-  return strip_geom(p->Nx, p->Ny, pick_seg_rows(p, nw), nw);
+  const int seg = pick_seg_rows(p, nw);
+  const StripGeom u = strip_geom(p->Nx, p->Ny, seg, nw);
+  return (u.nblocks >= 2048) ? strip_geom_banded(p->Nx, p->Ny, seg, nw) : u;
 }
 
 }  // namespace fv2d

@@ -82,11 +82,23 @@ struct BatchItem {
 };
 
 // Strip / segment geometry shared by the launcher, the kernel and the emulation.
+//  sched 0 (batch work items, small grids): uniform segments of `seg` rows; block b is
+//    (strip, segment) item strip_of_block(b).
+//  sched 1 (single grids of two or more rounds of workgroups): block b runs on XCD b % 8
+//    (the hardware's round-robin) as that XCD's (b / 8)-th block. XCD x owns the row band
+//    [x * nchunks / 8, (x + 1) * nchunks / 8) of chunks; its blocks, in dispatch order, take
+//    the band's long segments (lseg chunks) top-down and then its last `tail` chunks as short
+//    segments (sseg chunks), each segment across all nsx strips. The last workgroups of a
+//    launch are then short ones, so the launch drains in a short workgroup's time.
 struct StripGeom {
-  int nsx, nsy;  // strips across x, segments along y
+  int nsx, nsy;  // strips across x, segments along y (sched 0)
   int seg;       // rows per segment (a multiple of nw keeps every chunk full)
   int nblocks;
   int nw;        // rows per chunk = waves per workgroup (a CHUNK_ROWS value)
+  int sched;     // 0 uniform, 1 banded long-then-short (above)
+  int nchunks;   // chunks (nw rows) covering Ny (sched 1)
+  int lseg, sseg, tail;  // long / short segment and tail heights in chunks (sched 1)
+  int per_xcd;   // blocks per XCD (sched 1); blocks past a band's segments are empty
 };
 
 constexpr int STRIP_OUT = 62;  // output columns per strip
@@ -105,6 +117,39 @@ FV2D_HD StripGeom strip_geom(int Nx, int Ny, int seg, int nw) {
   g.nsx = (Nx + STRIP_OUT - 1) / STRIP_OUT;
   g.nsy = (Ny + seg - 1) / seg;
   g.nblocks = g.nsx * g.nsy;
+  g.sched = 0;
+  g.nchunks = g.lseg = g.sseg = g.tail = g.per_xcd = 0;
+  return g;
+}
+
+// Band of XCD x in chunks, and the number of its long and short segments (sched 1).
+FV2D_HD void xcd_band(const StripGeom &g, int x, int &c0, int &c1, int &nl, int &ns) {
+  c0 = (int)((long long)x * g.nchunks / 8);
+  c1 = (int)((long long)(x + 1) * g.nchunks / 8);
+  const int h = c1 - c0, t = h < g.tail ? h : g.tail;
+  nl = (h - t + g.lseg - 1) / g.lseg;
+  ns = (t + g.sseg - 1) / g.sseg;
+}
+
+// The banded long-then-short schedule (sched 1) for a grid whose uniform segmentation
+// already spans at least two rounds of workgroups: long segments of `seg` rows, short ones
+// of a quarter of that but at least two chunks (a short segment still pays the 4-row
+// prologue), a tail of one long segment per band.
+FV2D_HD StripGeom strip_geom_banded(int Nx, int Ny, int seg, int nw) {
+  StripGeom g = strip_geom(Nx, Ny, seg, nw);
+  g.sched = 1;
+  g.nchunks = (Ny + nw - 1) / nw;
+  g.lseg = (seg + nw - 1) / nw;
+  g.sseg = g.lseg >= 8 ? g.lseg / 4 : (g.lseg < 2 ? g.lseg : 2);
+  g.tail = g.lseg;
+  g.per_xcd = 0;
+  for (int x = 0; x < 8; ++x) {
+    int c0, c1, nl, ns;
+    xcd_band(g, x, c0, c1, nl, ns);
+    const int n = g.nsx * (nl + ns);
+    if (n > g.per_xcd) g.per_xcd = n;
+  }
+  g.nblocks = 8 * g.per_xcd;
   return g;
 }
 
@@ -118,6 +163,36 @@ FV2D_HD void strip_of_block(const StripGeom &g, int blk, int &sx, int &sy) {
   const int logical = x * q + (x < r ? x : r) + k;
   sx = logical % g.nsx;
   sy = logical / g.nsx;
+}
+
+// Workgroup id -> strip sx and its output rows [jb, je) (je == jb: nothing to do).
+FV2D_HD void block_rows(const StripGeom &g, int Ny, int blk, int &sx, int &jb, int &je) {
+  if (g.sched == 0) {
+    int sy;
+    strip_of_block(g, blk, sx, sy);
+    jb = sy * g.seg;
+    je = (jb + g.seg < Ny) ? jb + g.seg : Ny;
+    return;
+  }
+  const int x = blk & 7, k = blk >> 3;
+  int c0, c1, nl, ns;
+  xcd_band(g, x, c0, c1, nl, ns);
+  const int m = k / g.nsx;
+  sx = k - m * g.nsx;
+  const int cl = c1 - (c1 - c0 < g.tail ? c1 - c0 : g.tail);  // first chunk of the tail
+  int a, b;
+  if (m < nl) {
+    a = c0 + m * g.lseg;
+    b = (a + g.lseg < cl) ? a + g.lseg : cl;
+  } else if (m < nl + ns) {
+    a = cl + (m - nl) * g.sseg;
+    b = (a + g.sseg < c1) ? a + g.sseg : c1;
+  } else {
+    a = b = 0;
+  }
+  jb = a * g.nw;
+  je = (b * g.nw < Ny) ? b * g.nw : Ny;
+  if (je < jb) je = jb;
 }
 
 // Raw values (U or Q0, storage type) of the next chunk's ring row and halo columns, loaded
@@ -150,9 +225,16 @@ struct LaneRegs {
 // XG: gravity, the well-balanced wall flux or wall conduction modes are on; without it
 // their code is compiled out.
 // NW_: rows per chunk = waves per workgroup (a chunk_rows_supported value).
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW_>
+// DIVF_: how the update phases divide by dx and dy (FP64): DIVF_RUNTIME picks the exact-multiply
+// or the general form per call from kp (emulation, batch kernels, FP32 form); the single-grid
+// FP64 kernels are built once with DIVF_EXACT (both cell sizes powers of two) and once with
+// DIVF_GENERAL, so each holds only the form it runs.
+enum : int { DIVF_RUNTIME = 0, DIVF_EXACT = 1, DIVF_GENERAL = 2 };
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW_,
+          int DIVF_ = DIVF_RUNTIME>
 struct StripStage {
   static constexpr int NW = NW_;
+  static constexpr int DIVF = DIVF_;
   static constexpr int NT = 64 * NW;
   static constexpr int HX = STRIP_W + 2;  // loaded columns: 64 lanes + 2 right halo columns
   // Ring rows. While chunk jc runs (rows jc .. jc+NW-1): primitives of rows jc-1 .. jc+NW+1,
@@ -461,14 +543,31 @@ struct StripStage {
     if (VISC) visc_yface(lane, j, jb, sm, kp);
   }
 
+  // Both cell sizes are powers of two (the FP64 divisions by dx and dy are exact multiplies):
+  // the update phases are then compiled with that mode fixed, which drops the mode tests of
+  // every division from the chunk loop. The FP32 form multiplies in every mode.
+  static FV2D_HD bool exact_dxdy(const KParams &kp) {
+    if (DIVF != DIVF_RUNTIME) return DIVF == DIVF_EXACT;
+    return sizeof(R) == 8 && kp.divx == DIV_EXACT && kp.divy == DIV_EXACT;
+  }
+
   // Update part 1: conduction and viscous terms of the x face (lane, lane+1) of row jc+w.
   static FV2D_HD void update_faces(int lane, int w, int jb, int jc, const Smem &sm, const KParams &kp,
                                    LaneRegs<R> &r) {
+    if (exact_dxdy(kp))
+      update_faces_m<true>(lane, w, jb, jc, sm, kp, r);
+    else
+      update_faces_m<false>(lane, w, jb, jc, sm, kp, r);
+  }
+  template <bool EXD>
+  static FV2D_HD void update_faces_m(int lane, int w, int jb, int jc, const Smem &sm, const KParams &kp,
+                                     LaneRegs<R> &r) {
     const int j = jc + w;
     if (TC) {
       const R kc = R(kp.kappa);
       const R kf = R(0.5) * (kc + kc);
-      r.tcx = Ops<R>::div_c(kf * (T(sm, j, lane + 1, jb) - T(sm, j, lane, jb)), R(kp.dx), R(kp.inv_dx), kp.divx);
+      r.tcx = Ops<R>::div_c(kf * (T(sm, j, lane + 1, jb) - T(sm, j, lane, jb)), R(kp.dx), R(kp.inv_dx),
+                            EXD ? int(DIV_EXACT) : kp.divx);
     }
     if (VISC) {
       const R mu = R(kp.mu), odx = R(kp.inv_dx), ody = R(kp.inv_dy);
@@ -521,6 +620,16 @@ struct StripStage {
   static FV2D_HD void update_cell(int lane, int w, int i0, int jb, int jc, int je, const Smem &sm,
                                   const KParams &kp, const StageIO<S> &io, double dt, double dt_dx, double dt_dy,
                                   LaneRegs<R> &r) {
+    if (exact_dxdy(kp))
+      update_cell_m<true>(lane, w, i0, jb, jc, je, sm, kp, io, dt, dt_dx, dt_dy, r);
+    else
+      update_cell_m<false>(lane, w, i0, jb, jc, je, sm, kp, io, dt, dt_dx, dt_dy, r);
+  }
+  template <bool EXD>
+  static FV2D_HD void update_cell_m(int lane, int w, int i0, int jb, int jc, int je, const Smem &sm,
+                                    const KParams &kp, const StageIO<S> &io, double dt, double dt_dx,
+                                    double dt_dy, LaneRegs<R> &r) {
+    const int divx = EXD ? int(DIV_EXACT) : kp.divx, divy = EXD ? int(DIV_EXACT) : kp.divy;
     const int j = jc + w, gi = i0 - 2 + lane;
     if (lane < 2 || gi >= kp.Nx || j >= je) return;
     const R rho = Q(sm, IR, j, lane, jb);
@@ -540,7 +649,7 @@ struct StripStage {
     }
 
     // x sweep (Update.h:171)
-    add_flux(acc, r.fxl, r.fx, dt, dt_dx, kp.dx, kp.inv_dx, kp.divx);
+    add_flux(acc, r.fxl, r.fx, dt, dt_dx, kp.dx, kp.inv_dx, divx);
     if (XG && kp.grav != GRAV_NONE) add_gravity(acc, rho, r.fxl[IR], r.fx[IR], dt, gX);
 
     // y sweep (Update.h:172)
@@ -566,7 +675,7 @@ struct StripStage {
           FR[3] = R(0);
         }
       }
-      add_flux(acc, FL, FR, dt, dt_dy, kp.dy, kp.inv_dy, kp.divy);
+      add_flux(acc, FL, FR, dt, dt_dy, kp.dy, kp.inv_dy, divy);
       if (XG && kp.grav != GRAV_NONE) add_gravity(acc, rho, FL[IR], FR[IR], dt, gY);
     }
 
@@ -578,18 +687,18 @@ struct StripStage {
       const R Tc = T(sm, j, lane, jb);
       R FL = r.tcxl;
       R FR = r.tcx;
-      const R FU = Ops<R>::div_c(kf * (Tc - T(sm, j - 1, lane, jb)), dy, R(kp.inv_dy), kp.divy);
-      const R FD = Ops<R>::div_c(kf * (T(sm, j + 1, lane, jb) - Tc), dy, R(kp.inv_dy), kp.divy);
+      const R FU = Ops<R>::div_c(kf * (Tc - T(sm, j - 1, lane, jb)), dy, R(kp.inv_dy), divy);
+      const R FD = Ops<R>::div_c(kf * (T(sm, j + 1, lane, jb) - Tc), dy, R(kp.inv_dy), divy);
       // wall modes replace the x fluxes on the first / last row (ThermalConduction.h:75-103)
       if (XG && j == 0 && kp.bctc_ymin != BCTC_NONE) {
         if (kp.bctc_ymin == BCTC_FIXED_TEMPERATURE)
-          FL = Ops<R>::div_c(kf * R(2) * (Tc - R(kp.bctc_ymin_value)), dy, R(kp.inv_dy), kp.divy);
+          FL = Ops<R>::div_c(kf * R(2) * (Tc - R(kp.bctc_ymin_value)), dy, R(kp.inv_dy), divy);
         else if (kp.bctc_ymin == BCTC_FIXED_GRADIENT)
           FL = kf * R(kp.bctc_ymin_value);
       }
       if (XG && j == kp.Ny - 1 && kp.bctc_ymax != BCTC_NONE) {
         if (kp.bctc_ymax == BCTC_FIXED_TEMPERATURE)
-          FR = Ops<R>::div_c(kf * R(2) * (R(kp.bctc_ymax_value) - Tc), dy, R(kp.inv_dy), kp.divy);
+          FR = Ops<R>::div_c(kf * R(2) * (R(kp.bctc_ymax_value) - Tc), dy, R(kp.inv_dy), divy);
         else if (kp.bctc_ymax == BCTC_FIXED_GRADIENT)
           FR = kf * R(kp.bctc_ymax_value);
       }

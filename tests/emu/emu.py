@@ -37,7 +37,29 @@ def load(sanitized: bool = False):
     lib.emu_markstein_mismatches.argtypes = [dp, C.c_longlong, C.c_double, C.c_int]
     lib.emu_div_mode.restype = C.c_int
     lib.emu_div_mode.argtypes = [C.c_double]
+    lib.emu_use_banded.argtypes = [C.c_void_p, C.c_int]
+    lib.emu_block_rows.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    lib.emu_block_rows.restype = C.c_int
+    lib.emu_sim_geom.argtypes = [C.POINTER(Params), C.POINTER(C.c_int)]
     return lib
+
+
+def block_rows(Nx: int, Ny: int, seg: int, nw: int, banded: bool, lib=None) -> np.ndarray:
+    """(sx, jb, je) of every block of the stage kernel's geometry, shape (nblocks, 3)."""
+    lib = lib or load()
+    n = lib.emu_block_rows(Nx, Ny, seg, nw, int(banded), None)
+    out = np.zeros(3 * n, dtype=np.int32)
+    lib.emu_block_rows(Nx, Ny, seg, nw, int(banded), out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out.reshape(n, 3)
+
+
+def sim_geom(params: Params, lib=None) -> dict:
+    """Geometry the single-grid kernel picks for params."""
+    lib = lib or load()
+    out = (C.c_int * 6)()
+    p = params.copy()
+    lib.emu_sim_geom(C.byref(p), out)
+    return dict(zip(("nblocks", "seg", "sched", "lseg", "sseg", "tail"), list(out)))
 
 
 def _dp(a):
@@ -64,6 +86,10 @@ class Emu:
             self.close()
         except Exception:
             pass
+
+    def use_banded(self, seg: int):
+        """Run the stage with the banded long-then-short schedule (long segments of seg rows)."""
+        self.lib.emu_use_banded(self.h, int(seg))
 
     def set_prim(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

@@ -194,6 +194,35 @@ void *emu_create(const fv2d_params *p, char *err) {
 
 void emu_destroy(void *h) { delete (Sim *)h; }
 
+// Replaces the simulation's geometry by the banded long-then-short schedule with long
+// segments of seg rows (whatever the grid size), so that small grids exercise it.
+void emu_use_banded(void *h, int seg) {
+  Sim &s = *(Sim *)h;
+  s.g = strip_geom_banded(s.p.Nx, s.p.Ny, seg, s.g.nw);
+  s.partials.assign((size_t)s.g.nblocks, 0.0);
+}
+
+// Geometry of block_rows for a test of the mapping: returns nblocks and fills, per block,
+// (sx, jb, je) into out (3 * nblocks ints, when out is not NULL).
+int emu_block_rows(int Nx, int Ny, int seg, int nw, int banded, int *out) {
+  const StripGeom g = banded ? strip_geom_banded(Nx, Ny, seg, nw) : strip_geom(Nx, Ny, seg, nw);
+  if (out)
+    for (int b = 0; b < g.nblocks; ++b) block_rows(g, Ny, b, out[3 * b], out[3 * b + 1], out[3 * b + 2]);
+  return g.nblocks;
+}
+
+// Geometry the single-grid kernel uses for these parameters: (nblocks, seg, sched, lseg,
+// sseg, tail) into out[6].
+void emu_sim_geom(const fv2d_params *p, int *out) {
+  const StripGeom g = sim_geom(p);
+  out[0] = g.nblocks;
+  out[1] = g.seg;
+  out[2] = g.sched;
+  out[3] = g.lseg;
+  out[4] = g.sseg;
+  out[5] = g.tail;
+}
+
 void emu_set_prim(void *h, const double *q) {
   Sim &s = *(Sim *)h;
   if (s.fp32) set_prim_t<float, float>(s, s.Uf, s.Q0f, q);

@@ -54,10 +54,9 @@ void run_stage(Sim &s, const StageIO<S> &io) {
   typename K::Smem *sm = (typename K::Smem *)malloc(sizeof(typename K::Smem));
   const R nan = std::numeric_limits<R>::quiet_NaN();
   for (int blk = 0; blk < s.g.nblocks; ++blk) {
-    int sx, sy;
-    strip_of_block(s.g, blk, sx, sy);
-    const int i0 = sx * STRIP_OUT, jb = sy * s.g.seg;
-    const int je = (jb + s.g.seg < kp.Ny) ? jb + s.g.seg : kp.Ny;
+    int sx, jb, je;
+    block_rows(s.g, kp.Ny, blk, sx, jb, je);
+    const int i0 = sx * STRIP_OUT;
     R *raw = (R *)sm;
     for (size_t k = 0; k < sizeof(typename K::Smem) / sizeof(R); ++k) raw[k] = nan;
     for (auto &r : regs) {
@@ -76,10 +75,12 @@ void run_stage(Sim &s, const StageIO<S> &io) {
   }
     std::vector<RowPrefetch<S>> pfs(K::NT);
     for (auto &p : pfs) memset(&p, 0xff, sizeof p);  // garbage until loaded
-    if (K::kPF && jb < je) LANES(K::chunk_prefetch(lane, w, 0, i0, jb, kp, io, pfs[t]));
-    LANES(K::pro_load(lane, w, i0, jb, *sm, kp, io));
-    LANES(K::pro_slopes(lane, w, jb, *sm));
-    LANES(K::pro_flux(lane, w, jb, *sm, kp));
+    if (jb < je) {
+      if (K::kPF) LANES(K::chunk_prefetch(lane, w, 0, i0, jb, kp, io, pfs[t]));
+      LANES(K::pro_load(lane, w, i0, jb, *sm, kp, io));
+      LANES(K::pro_slopes(lane, w, jb, *sm));
+      LANES(K::pro_flux(lane, w, jb, *sm, kp));
+    }
     int xw = 0;
     for (int jc = jb; jc < je; jc += K::NW) {
       if (K::kPF) {

@@ -28,8 +28,10 @@ def make(n, bcx, bcy, recon, riem, ts, tc, vi, grav, wb, Nx=37, Ny=19):
     return p, P.random_state(p, seed=n)
 
 
-def compare(p, q0, lib, oracle_lib, steps=3):
+def compare(p, q0, lib, oracle_lib, steps=3, banded=0):
     o, e = Oracle(p, oracle_lib), EMU.Emu(p, lib)
+    if banded:
+        e.use_banded(banded)
     o.set_prim(q0)
     e.set_prim(q0)
     for s in range(steps):
@@ -83,6 +85,29 @@ def test_chunk_heights_bitwise(nw, Nx, Ny, seg, emu_lib, oracle_lib):
         fp32_close(emu_lib, oracle_lib, N=48, steps=6, chunk_rows=nw)
 
 
+@pytest.mark.parametrize("xr,yr", [(1.0, 1.0), (0.5, 2.0), (1.0, 0.7), (0.9, 1.0)])
+def test_power_of_two_cell_sizes_bitwise(xr, yr, emu_lib, oracle_lib):
+    """Cell sizes that are powers of two in both directions (the update phases are compiled
+    with the divisions by dx and dy as exact multiplies) and in one direction only (the
+    general form, one direction exact at run time); conduction with wall modes, viscosity
+    and gravity on."""
+    for n, c in enumerate([(1, 1, PLM, 1, 1, 1, 1, 1, 1), (0, 1, PLM, 2, 1, 1, 1, 2, 1), (2, 0, PLM, 0, 0, 1, 0, 0, 0)]):
+        p, _ = make(n, *c, Nx=32, Ny=16)
+        p = p.copy(xmin=0.0, xmax=xr, ymin=0.0, ymax=yr)
+        compare(p, P.random_state(p, seed=n), emu_lib, oracle_lib, steps=3)
+
+
+@pytest.mark.parametrize("Nx,Ny,seg,nw", [(124, 150, 8, 4), (63, 301, 12, 4), (130, 37, 4, 4), (37, 19, 8, 4),
+                                          (124, 200, 16, 8), (5, 3, 8, 4), (70, 64, 32, 4)])
+def test_banded_schedule_bitwise(Nx, Ny, seg, nw, emu_lib, oracle_lib):
+    """The banded long-then-short block schedule of large single grids, forced on small ones:
+    bands of zero to many chunks, long segments cut short by the tail, one-chunk and
+    two-chunk short segments, padding blocks (dt partial zero), 8-row chunks."""
+    for n, c in enumerate([(1, 1, PLM, 1, 1, 1, 1, 1, 1), (2, 0, PLM, 0, 0, 0, 1, 0, 0), (0, 2, 0, 2, 1, 1, 0, 2, 1)]):
+        p, q0 = make(n, *c, Nx=Nx, Ny=Ny)
+        compare(p.copy(chunk_rows=nw), q0, emu_lib, oracle_lib, steps=2, banded=seg)
+
+
 def test_sanitized_bounds(oracle_lib):
     """ASan/UBSan build: every 9th combination and three strip / segment edge shapes, 2 steps;
     a bad index aborts the process, so this test runs the sanitized library in a child
@@ -103,6 +128,9 @@ def test_sanitized_bounds(oracle_lib):
         "                        (5, 3, 0, 8)):\n"
         "    p, q0 = T.make(0, 1, 1, 2, 1, 1, 1, 1, 1, 1, Nx=Nx, Ny=Ny)\n"
         "    T.compare(p.copy(seg_rows=seg, chunk_rows=nw), q0, lib, olib, steps=2)\n"
+        "for Nx, Ny, seg, nw in ((124, 150, 8, 4), (5, 3, 8, 4), (124, 200, 16, 8)):\n"
+        "    p, q0 = T.make(0, 1, 1, 2, 1, 1, 1, 1, 1, 1, Nx=Nx, Ny=Ny)\n"
+        "    T.compare(p.copy(chunk_rows=nw), q0, lib, olib, steps=2, banded=seg)\n"
         "T.fp32_close(lib, olib, N=24, steps=4)\n"
         "print('ok')\n" % ([os.path.dirname(__file__), os.path.dirname(os.path.dirname(__file__)),
                            os.path.join(os.path.dirname(__file__), "emu")],))

@@ -223,3 +223,19 @@ def test_chunk_heights_workloads(nw, oracle_lib, gpu_lib):
     for name, size in (("kh", dict(N=128)), ("rt", dict(Nx=64, Ny=192))):
         p, q0, _ = P.workload(name, **size)
         assert run_pair(p.copy(chunk_rows=nw), q0, 10, oracle_lib, gpu_lib, every=5)
+
+
+@pytest.mark.parametrize("Nx,Ny,wall", [(4096, 1100, 0), (2100, 2049, 1)])
+def test_banded_schedule_general_division(Nx, Ny, wall, oracle_lib, gpu_lib):
+    """Grids large enough for the banded long-then-short block schedule, with cell sizes that
+    are not powers of two (the general division path of the update phases): PLM / HLLC /
+    RK2 with conduction and viscosity, periodic or reflecting walls with gravity, the
+    well-balanced wall flux and both wall conduction modes; 3 steps, every step compared."""
+    p = Params.defaults(Nx=Nx, Ny=Ny, xmin=0.1, xmax=1.3, ymin=-0.2, ymax=0.7, gamma0=1.4, CFL=0.4,
+                        reconstruction=PLM, riemann_solver=HLLC, tc_active=1, kappa=3e-4, visc_active=1,
+                        mu=2e-4, boundary_x=BC_PERIODIC, boundary_y=BC_REFLECTING if wall else BC_PERIODIC,
+                        gravity_mode=GRAV_CONSTANT if wall else GRAV_NONE, gy=-0.7,
+                        well_balanced_flux_at_y_bc=wall, bctc_ymin=1 if wall else 0, bctc_ymax=2 if wall else 0,
+                        bctc_ymin_value=1.5, bctc_ymax_value=0.3)
+    assert run_pair(p, P.random_state(p, seed=Nx + Ny), 3, oracle_lib, gpu_lib)
+
