@@ -259,7 +259,6 @@ int fv2d_set_problem(fv2d_handle *h, const fv2d_problem *prob) {
                        (double *)h->d.U, (double *)h->d.Q0);
   HIPH(h, hipGetLastError());
   hipLaunchKernelGGL(clear_ctl_kernel, dim3(1), dim3(1), 0, h->d.stream, h->d.ctl);
-  int rc;
   SIMH(h, sim_launch_dt_tiles(h->d, h->d.Q0, 1));
   SIMH(h, sim_launch_reduce(h->d, 0, nullptr));
   HIPH(h, hipStreamSynchronize(h->d.stream));

@@ -63,7 +63,7 @@ __device__ inline void commit_epilogue(unsigned nr, unsigned np, unsigned nn, do
 // loop.
 template <class K, typename R, typename S>
 __device__ inline void strip_rows(const KParams &kp, const StageIO<S> &io, int i0, int jb, int je, double dt,
-                                  double dt_dx, double dt_dy, typename K::Smem &sm, LaneRegs<R> &r) {
+                                  double dt_dx, double dt_dy, typename K::Smem &sm, typename K::Regs &r) {
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: ring slots in SGPRs
   RowPrefetch<S> pf;
@@ -118,7 +118,7 @@ __device__ inline void strip_block(const KParams &kp, const StageIO<S> &io, cons
   block_rows(g, kp.Ny, blk, sx, jb, je);
   const int i0 = sx * STRIP_OUT;
 
-  LaneRegs<R> r;
+  typename K::Regs r;
   r.neg_rho = r.neg_prs = r.nan_cnt = 0;
   r.inv = 0.0;
 

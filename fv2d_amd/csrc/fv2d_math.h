@@ -448,16 +448,22 @@ FV2D_HD int map_ghost(int g, int N, int bc, bool &flip) {
   return s;
 }
 
-// The hyperbolic inverse time step of one cell (ComputeDt.h:31-34), given its sound speed.
+// The hyperbolic inverse time step of one cell (ComputeDt.h:31-34), given its sound speed;
+// divx / divy: the division modes of dx and dy (kp.divx / kp.divy, or those values known at
+// compile time).
+template <typename R>
+FV2D_HD double inv_dt_hyp_cs(const St<R> &q, R cs, const KParams &kp, int divx, int divy) {
+  if (sizeof(R) == 8) {
+    return markstein_div((double)(cs + fabs(q.v[IU])), kp.dx, kp.inv_dx, divx) +
+           markstein_div((double)(cs + fabs(q.v[IV])), kp.dy, kp.inv_dy, divy);
+  } else {
+    // FP32 form: the per-cell value in FP32, the reduction over cells in FP64
+    return (double)((cs + fabs(q.v[IU])) * R(kp.inv_dx) + (cs + fabs(q.v[IV])) * R(kp.inv_dy));
+  }
+}
 template <typename R>
 FV2D_HD double inv_dt_hyp_cs(const St<R> &q, R cs, const KParams &kp) {
-  if (sizeof(R) == 8) {
-    return markstein_div((double)(cs + fabs(q.v[IU])), kp.dx, kp.inv_dx, kp.divx) +
-           markstein_div((double)(cs + fabs(q.v[IV])), kp.dy, kp.inv_dy, kp.divy);
-  } else {
-    // FP32 form: FP64 for the reduction, per-cell value from the FP32 state
-    return ((double)cs + fabs((double)q.v[IU])) * kp.inv_dx + ((double)cs + fabs((double)q.v[IV])) * kp.inv_dy;
-  }
+  return inv_dt_hyp_cs(q, cs, kp, kp.divx, kp.divy);
 }
 
 template <typename R>

@@ -204,11 +204,11 @@ struct RowPrefetch {
   int flags;  // ghost sign flips: bit 0/1 row u/v, bit 2/3 extra u/v
 };
 
-// Per-lane state carried between phases (registers on the GPU).
-template <typename R>
+// Per-lane state carried between phases (registers on the GPU). S: storage type of U.
+template <typename R, typename S = R>
 struct LaneRegs {
-  double u[4];   // own cell U (base of the update)
-  double u0[4];  // own cell U^n (RK2 average)
+  S u[4];        // own cell U (base of the update)
+  S u0[4];       // own cell U^n (RK2 average)
   R sx[4];       // x slope of strip column l+1
   R sxl[4];      // x slope of column l (from lane l-1)
   R syu[4];      // y slope of (l, row+1)
@@ -257,6 +257,7 @@ struct StripStage {
   static constexpr bool kPF = sizeof(S) == 4 || FV2D_PREFETCH_FP64;
   // the halo columns too (FP32: 4 more registers; FP64 loads them in chunk_commit)
   static constexpr bool kPFX = kPF && sizeof(S) == 4;
+  typedef LaneRegs<R, S> Regs;
 
   struct Smem {
     R q[4][NQ][HX];
@@ -357,15 +358,15 @@ struct StripStage {
   }
 
   // Own-cell U (and U^n) of output lanes: issued at the start of a chunk, used by update.
-  static FV2D_HD void load_own(int lane, int j, int i0, const KParams &kp, const StageIO<S> &io, LaneRegs<R> &r) {
+  static FV2D_HD void load_own(int lane, int j, int i0, const KParams &kp, const StageIO<S> &io, Regs &r) {
     const int gi = i0 - 2 + lane;
     if (lane >= 2 && gi < kp.Nx && j < kp.Ny) {
       const long long k = (long long)j * kp.pitch + gi;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) r.u[v] = (double)io.src[v * kp.plane + k];
+      for (int v = 0; v < 4; ++v) r.u[v] = io.src[v * kp.plane + k];
       if (io.avg) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) r.u0[v] = (double)io.avg[v * kp.plane + k];
+        for (int v = 0; v < 4; ++v) r.u0[v] = io.avg[v * kp.plane + k];
       }
     }
   }
@@ -459,7 +460,7 @@ struct StripStage {
   // ---------------------------------------------------------------- chunk phases
   // Rows jc+2 .. jc+NW+1 (wave w: jc+2+w); the right halo columns by wave `xw`.
   static FV2D_HD void chunk_load(int lane, int w, int xw, int i0, int jb, int jc, Smem &sm, const KParams &kp,
-                                 const StageIO<S> &io, LaneRegs<R> &r) {
+                                 const StageIO<S> &io, Regs &r) {
     load_row(lane, w, jc + 2 + w, i0, jb, sm, kp, io);
     if (w == xw) load_extra(lane, jc + 2, NW, i0, jb, sm, kp, io);
     load_own(lane, jc + w, i0, kp, io, r);
@@ -478,7 +479,7 @@ struct StripStage {
     }
   }
   static FV2D_HD void chunk_commit(int lane, int w, int xw, int i0, int jb, int jc, Smem &sm, const KParams &kp,
-                                   const StageIO<S> &io, const RowPrefetch<S> &p, LaneRegs<R> &r) {
+                                   const StageIO<S> &io, const RowPrefetch<S> &p, Regs &r) {
     St<R> q;
     R t = R(0);
     raw_to_q(p.row, (p.flags & 1) != 0, (p.flags & 2) != 0, kp, io, q, t);
@@ -497,7 +498,7 @@ struct StripStage {
   }
 
   // Pass 1a: x slope of column lane+1 of row jc+w, and the y slope of (lane, jc+w+1).
-  static FV2D_HD void pass1_slopes(int lane, int w, int jb, int jc, Smem &sm, LaneRegs<R> &r) {
+  static FV2D_HD void pass1_slopes(int lane, int w, int jb, int jc, Smem &sm, Regs &r) {
     const int j = jc + w;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
@@ -510,7 +511,7 @@ struct StripStage {
 
   // Pass 1b: the x interface (lane, lane+1) of row jc+w (needs sxl = slope of column lane).
   static FV2D_HD void pass1_flux(int lane, int w, int jb, int jc, const Smem &sm, const KParams &kp,
-                                 LaneRegs<R> &r) {
+                                 Regs &r) {
     const int j = jc + w;
     St<R> a, b;
 #pragma unroll
@@ -527,7 +528,7 @@ struct StripStage {
   }
 
   // Pass 2: the y interface (jc+w, jc+w+1) at column lane, and its viscous terms.
-  static FV2D_HD void pass2_flux(int lane, int w, int jb, int jc, Smem &sm, const KParams &kp, LaneRegs<R> &r) {
+  static FV2D_HD void pass2_flux(int lane, int w, int jb, int jc, Smem &sm, const KParams &kp, Regs &r) {
     const int j = jc + w;
     R slo[4];
 #pragma unroll
@@ -553,7 +554,7 @@ struct StripStage {
 
   // Update part 1: conduction and viscous terms of the x face (lane, lane+1) of row jc+w.
   static FV2D_HD void update_faces(int lane, int w, int jb, int jc, const Smem &sm, const KParams &kp,
-                                   LaneRegs<R> &r) {
+                                   Regs &r) {
     if (exact_dxdy(kp))
       update_faces_m<true>(lane, w, jb, jc, sm, kp, r);
     else
@@ -561,7 +562,7 @@ struct StripStage {
   }
   template <bool EXD>
   static FV2D_HD void update_faces_m(int lane, int w, int jb, int jc, const Smem &sm, const KParams &kp,
-                                     LaneRegs<R> &r) {
+                                     Regs &r) {
     const int j = jc + w;
     if (TC) {
       const R kc = R(kp.kappa);
@@ -619,7 +620,7 @@ struct StripStage {
   // Update part 2: the cell (lane, jc+w) (needs fxl, tcxl, vxl from lane-1).
   static FV2D_HD void update_cell(int lane, int w, int i0, int jb, int jc, int je, const Smem &sm,
                                   const KParams &kp, const StageIO<S> &io, double dt, double dt_dx, double dt_dy,
-                                  LaneRegs<R> &r) {
+                                  Regs &r) {
     if (exact_dxdy(kp))
       update_cell_m<true>(lane, w, i0, jb, jc, je, sm, kp, io, dt, dt_dx, dt_dy, r);
     else
@@ -628,7 +629,7 @@ struct StripStage {
   template <bool EXD>
   static FV2D_HD void update_cell_m(int lane, int w, int i0, int jb, int jc, int je, const Smem &sm,
                                     const KParams &kp, const StageIO<S> &io, double dt, double dt_dx,
-                                    double dt_dy, LaneRegs<R> &r) {
+                                    double dt_dy, Regs &r) {
     const int divx = EXD ? int(DIV_EXACT) : kp.divx, divy = EXD ? int(DIV_EXACT) : kp.divy;
     const int j = jc + w, gi = i0 - 2 + lane;
     if (lane < 2 || gi >= kp.Nx || j >= je) return;
@@ -723,23 +724,34 @@ struct StripStage {
       acc[IE] = acc[IE] + A(fxe + fye) * A(dt);
     }
 
-    double un64[4];
+    S uns[4];
+    if (sizeof(R) == 8 || sizeof(S) == 8) {
+      double un64[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) un64[v] = (sizeof(R) == 8) ? double(acc[v]) : r.u[v] + double(acc[v]);
-
-    // SSP-RK2 average (Update.h:214-220)
-    if (io.avg) {
+      for (int v = 0; v < 4; ++v) un64[v] = (sizeof(R) == 8) ? double(acc[v]) : double(r.u[v]) + double(acc[v]);
+      // SSP-RK2 average (Update.h:214-220)
+      if (io.avg) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) un64[v] = 0.5 * (r.u0[v] + un64[v]);
+        for (int v = 0; v < 4; ++v) un64[v] = 0.5 * (double(r.u0[v]) + un64[v]);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) uns[v] = S(un64[v]);
+    } else {
+      // FP32 state: U + dt*dF and the RK2 average in FP32, each rounded once, like the storage
+#pragma unroll
+      for (int v = 0; v < 4; ++v) uns[v] = S(r.u[v] + S(acc[v]));
+      if (io.avg) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) uns[v] = S(0.5f) * (r.u0[v] + uns[v]);
+      }
     }
 
     const long long k = (long long)j * kp.pitch + gi;
     St<R> un;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const S s = S(un64[v]);
-      io.out[v * kp.plane + k] = s;
-      un.v[v] = R(s);
+      io.out[v * kp.plane + k] = uns[v];
+      un.v[v] = R(uns[v]);
     }
 
     // epilogue: consToPrim + checkNegatives (SimInfo.h:576-646) and the next dt's cell term
@@ -763,7 +775,7 @@ struct StripStage {
         c2 = byr(q.v[IP] * R(kp.gamma0));
       else
         c2 = Ops<R>::div(q.v[IP] * R(kp.gamma0), q.v[IR]);
-      const double inv = inv_dt_hyp_cs(q, Ops<R>::sqrt_(c2), kp);
+      const double inv = inv_dt_hyp_cs(q, Ops<R>::sqrt_(c2), kp, divx, divy);
       if (inv > r.inv) r.inv = inv;  // NaN never wins: the reference's fmax-based Max ignores it
     }
   }
