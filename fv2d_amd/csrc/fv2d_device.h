@@ -79,9 +79,11 @@ __device__ inline void strip_rows(const KParams &kp, const StageIO<S> &io, int i
   for (int jc = jb; jc < je; jc += K::NW) {
     if (K::kPF) {
       K::chunk_commit(lane, w, xw, i0, jb, jc, sm, kp, io, pf, r);
+      if (!K::kLateOwn) K::load_own(lane, jc + w, i0, kp, io, r);
       if (jc + K::NW < je) K::chunk_prefetch(lane, w, (xw + 1) % K::NW, i0, jc + K::NW, kp, io, pf);
     } else {
       K::chunk_load(lane, w, xw, i0, jb, jc, sm, kp, io, r);
+      if (!K::kLateOwn) K::load_own(lane, jc + w, i0, kp, io, r);
     }
     __syncthreads();
     K::pass1_slopes(lane, w, jb, jc, sm, r);
@@ -94,6 +96,7 @@ __device__ inline void strip_rows(const KParams &kp, const StageIO<S> &io, int i
     K::pass1_flux(lane, w, jb, jc, sm, kp, r);
     K::pass2_flux(lane, w, jb, jc, sm, kp, r);
     K::update_faces(lane, w, jb, jc, sm, kp, r);
+    if (K::kLateOwn) K::load_own(lane, jc + w, i0, kp, io, r);
 #pragma unroll
     for (int v = 0; v < 4; ++v) r.fxl[v] = lane_shr1(r.fx[v]);
     if (K::kTC) r.tcxl = lane_shr1(r.tcx);

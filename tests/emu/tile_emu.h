@@ -85,10 +85,12 @@ void run_stage(Sim &s, const StageIO<S> &io) {
     for (int jc = jb; jc < je; jc += K::NW) {
       if (K::kPF) {
         LANES(K::chunk_commit(lane, w, xw, i0, jb, jc, *sm, kp, io, pfs[t], r));
+        if (!K::kLateOwn) LANES(K::load_own(lane, jc + w, i0, kp, io, r));
         if (jc + K::NW < je)
           LANES(K::chunk_prefetch(lane, w, (xw + 1) % K::NW, i0, jc + K::NW, kp, io, pfs[t]));
       } else {
         LANES(K::chunk_load(lane, w, xw, i0, jb, jc, *sm, kp, io, r));
+        if (!K::kLateOwn) LANES(K::load_own(lane, jc + w, i0, kp, io, r));
       }
       LANES(K::pass1_slopes(lane, w, jb, jc, *sm, r));
       shift(regs, [](LaneRegs<R> &d, const LaneRegs<R> *s_) {
@@ -97,6 +99,7 @@ void run_stage(Sim &s, const StageIO<S> &io) {
       LANES(K::pass1_flux(lane, w, jb, jc, *sm, kp, r));
       LANES(K::pass2_flux(lane, w, jb, jc, *sm, kp, r));
       LANES(K::update_faces(lane, w, jb, jc, *sm, kp, r));
+      if (K::kLateOwn) LANES(K::load_own(lane, jc + w, i0, kp, io, r));
       shift(regs, [](LaneRegs<R> &d, const LaneRegs<R> *s_) {
         for (int v = 0; v < 4; ++v) d.fxl[v] = s_ ? s_->fx[v] : R(0);
         if (K::kTC) d.tcxl = s_ ? s_->tcx : R(0);
