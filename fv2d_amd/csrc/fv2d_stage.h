@@ -81,6 +81,39 @@ struct BatchItem {
   int blk;  // workgroup item of the job
 };
 
+// Accessor of one row that is the same for the whole wave (a ring row, the own cells, the
+// output row), read or written at per-lane columns. On the GPU the row address goes into a
+// raw buffer resource (scalar registers: untyped, stride 0, 2^31 - 1 bytes, the gfx9
+// 32-bit data-format word) and each lane supplies a 32-bit byte offset, so the four fields
+// of a cell cost no vector address arithmetic; on the host it is a plain pointer. The row
+// must be wave-uniform, and its columns within 2^28 of it.
+template <typename T>
+struct URow {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __amdgpu_buffer_rsrc_t r;
+  __device__ explicit URow(const T *row)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(row), 0, 0x7fffffff, 0x00020000)) {}
+  __device__ T operator[](int col) const {
+    if constexpr (sizeof(T) == 4)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, col * 4, 0, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, col * 8, 0, 0));
+  }
+  __device__ void store(int col, T x) const {
+    if constexpr (sizeof(T) == 4)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, x), r, col * 4, 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, col * 8, 0, 0);
+  }
+#else
+  T *p;
+  explicit URow(const T *row) : p(const_cast<T *>(row)) {}
+  T operator[](int col) const { return p[col]; }
+  void store(int col, T x) const { p[col] = x; }
+#endif
+};
+
 // Strip / segment geometry shared by the launcher, the kernel and the emulation.
 //  sched 0 (batch work items, small grids): uniform segments of `seg` rows; block b is
 //    (strip, segment) item strip_of_block(b).
@@ -276,17 +309,24 @@ struct StripStage {
   static FV2D_HD int vslot(int j, int jb) { return (j - jb + 1) % NV; }
 
   // Storage values of domain-relative cell (gi, gj), ghosts included (mapped to their source
-  // cell; fx / fy: the normal velocity changes sign).
+  // cell; fx / fy: the normal velocity changes sign). UNI: gj is the same for the whole wave.
+  template <bool UNI>
   static FV2D_HD void fetch_raw(int gi, int gj, const KParams &kp, const StageIO<S> &io, S raw[4], bool &fx,
                                 bool &fy) {
     fx = false;
     fy = false;
     const int si = (gi >= 0 && gi < kp.Nx) ? gi : map_ghost(gi, kp.Nx, kp.bcx, fx);
     const int sj = (gj >= 0 && gj < kp.Ny) ? gj : map_ghost(gj, kp.Ny, kp.bcy, fy);
-    const long long k = (long long)sj * kp.pitch + si;
     const S *base = (io.qsrc == QSRC_Q0) ? io.q0 : io.src;
+    if (UNI) {
+      const S *row = base + (long long)sj * kp.pitch;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) raw[v] = base[v * kp.plane + k];
+      for (int v = 0; v < 4; ++v) raw[v] = URow<S>(row + (long long)v * kp.plane)[si];
+    } else {
+      const long long k = (long long)sj * kp.pitch + si;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) raw[v] = base[v * kp.plane + k];
+    }
   }
 
   // Primitives from storage values: Q0 as given, else consToPrim (+ reset); with conduction
@@ -317,10 +357,11 @@ struct StripStage {
   }
 
   // Primitive state (and T) of domain-relative cell (gi, gj), ghosts included.
+  template <bool UNI>
   static FV2D_HD void fetch_q(int gi, int gj, const KParams &kp, const StageIO<S> &io, St<R> &q, R &t) {
     S raw[4];
     bool fx, fy;
-    fetch_raw(gi, gj, kp, io, raw, fx, fy);
+    fetch_raw<UNI>(gi, gj, kp, io, raw, fx, fy);
     raw_to_q(raw, fx, fy, kp, io, q, t);
   }
 
@@ -342,7 +383,7 @@ struct StripStage {
                                const StageIO<S> &io) {
     St<R> q;
     R t = R(0);
-    fetch_q(i0 - 2 + lane, j, kp, io, q, t);
+    fetch_q<true>(i0 - 2 + lane, j, kp, io, q, t);
     put(sm, qslot(j, jb), lane, q, t);
     (void)w;
   }
@@ -352,7 +393,7 @@ struct StripStage {
       const int j = j0 + (lane >> 1), col = STRIP_W + (lane & 1);
       St<R> q;
       R t = R(0);
-      fetch_q(i0 - 2 + col, j, kp, io, q, t);
+      fetch_q<false>(i0 - 2 + col, j, kp, io, q, t);
       put(sm, qslot(j, jb), col, q, t);
     }
   }
@@ -366,12 +407,12 @@ struct StripStage {
   static FV2D_HD void load_own(int lane, int j, int i0, const KParams &kp, const StageIO<S> &io, Regs &r) {
     const int gi = i0 - 2 + lane;
     if (lane >= 2 && gi < kp.Nx && j < kp.Ny) {
-      const long long k = (long long)j * kp.pitch + gi;
+      const long long k = (long long)j * kp.pitch;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) r.u[v] = io.src[v * kp.plane + k];
+      for (int v = 0; v < 4; ++v) r.u[v] = URow<S>(io.src + (k + (long long)v * kp.plane))[gi];
       if (io.avg) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) r.u0[v] = io.avg[v * kp.plane + k];
+        for (int v = 0; v < 4; ++v) r.u0[v] = URow<S>(io.avg + (k + (long long)v * kp.plane))[gi];
       }
     }
   }
@@ -475,10 +516,10 @@ struct StripStage {
   static FV2D_HD void chunk_prefetch(int lane, int w, int xwn, int i0, int jn, const KParams &kp,
                                      const StageIO<S> &io, RowPrefetch<S> &p) {
     bool fx, fy;
-    fetch_raw(i0 - 2 + lane, jn + 2 + w, kp, io, p.row, fx, fy);
+    fetch_raw<true>(i0 - 2 + lane, jn + 2 + w, kp, io, p.row, fx, fy);
     p.flags = (fx ? 1 : 0) | (fy ? 2 : 0);
     if (kPFX && w == xwn && lane < 2 * NW) {
-      fetch_raw(i0 - 2 + STRIP_W + (lane & 1), jn + 2 + (lane >> 1), kp, io, p.extra, fx, fy);
+      fetch_raw<false>(i0 - 2 + STRIP_W + (lane & 1), jn + 2 + (lane >> 1), kp, io, p.extra, fx, fy);
       p.flags |= (fx ? 4 : 0) | (fy ? 8 : 0);
     }
   }
@@ -749,11 +790,11 @@ struct StripStage {
       }
     }
 
-    const long long k = (long long)j * kp.pitch + gi;
+    const long long k = (long long)j * kp.pitch;
     St<R> un;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      io.out[v * kp.plane + k] = uns[v];
+      URow<S>(io.out + (k + (long long)v * kp.plane)).store(gi, uns[v]);
       un.v[v] = R(uns[v]);
     }
 
