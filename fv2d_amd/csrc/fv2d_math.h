@@ -259,6 +259,17 @@ FV2D_HD R minmod(R dL, R dR) {
   if (dL * dR < R(0)) return R(0);
   return (fabs(dL) < fabs(dR)) ? dL : dR;
 }
+// FP32 form: the median of (dL, dR, 0), one v_med3_f32. It is the value above except for the
+// sign of a zero result and for a product that underflows to zero (the median is then 0,
+// not the smaller magnitude); the parity form keeps the reference's expression.
+template <>
+FV2D_HD float minmod(float dL, float dR) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_fmed3f(dL, dR, 0.0f);
+#else
+  return fmaxf(fminf(dL, dR), fminf(fmaxf(dL, dR), 0.0f));
+#endif
+}
 
 // PLM face value q + ((slope * sign) * 0.5) (Update.h:25, via the State operators
 // States.h:64-92). sign = +1 for the right face, -1 for the left face.
