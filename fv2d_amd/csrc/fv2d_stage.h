@@ -753,7 +753,19 @@ struct StripStage {
     // viscosity (Viscosity.h:33-117): every face term is computed once (x faces by the lane
     // to the left of the face, y faces in pass 2); per cell and direction the flux is
     // 0 + (-mu)*tau(low face) + (+mu)*tau(high face), as the reference accumulates it
-    if (VISC) {
+    if (VISC && sizeof(R) == 4) {
+      // FP32 form: the same face terms, each difference taken once (no zero-based
+      // accumulation, no zero density term)
+      const int lo = vslot(j - 1, jb), hi = vslot(j, jb);
+      const R fu = (r.vx[0] - r.vxl[0]) + (sm.vy[0][VISC ? hi : 0][lane] - sm.vy[0][VISC ? lo : 0][lane]);
+      const R fv = (r.vx[1] - r.vxl[1]) +
+                   (sm.vy[VISC ? 1 : 0][VISC ? hi : 0][lane] - sm.vy[VISC ? 1 : 0][VISC ? lo : 0][lane]);
+      const R fe = (r.vx[2] - r.vxl[2]) +
+                   (sm.vy[VISC ? 2 : 0][VISC ? hi : 0][lane] - sm.vy[VISC ? 2 : 0][VISC ? lo : 0][lane]);
+      acc[IU] = acc[IU] + A(fu) * A(dt);
+      acc[IV] = acc[IV] + A(fv) * A(dt);
+      acc[IE] = acc[IE] + A(fe) * A(dt);
+    } else if (VISC) {
       const int lo = vslot(j - 1, jb), hi = vslot(j, jb);
       const R fxu = (R(0) + (-r.vxl[0])) + r.vx[0];
       const R fxv = (R(0) + (-r.vxl[1])) + r.vx[1];
