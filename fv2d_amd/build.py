@@ -34,6 +34,7 @@ COMMON = [
 # (source, extra flags)
 UNITS = [
     ("stage_fp64.hip", ["-ffp-contract=off"]),
+    ("stage_fp64_exact.hip", ["-ffp-contract=off"]),
     ("stage_fp32.hip", ["-ffp-contract=fast"]),
     ("fv2d_capi.hip", ["-ffp-contract=off"]),
     ("fv2d_batch.hip", ["-ffp-contract=off"]),
@@ -56,7 +57,10 @@ def _compile(unit, defines=(), objdir=OBJ):
     return obj, noise
 
 
-def build(jobs: int = 3, verbose: bool = False, defines=(), out: str = LIB) -> str:
+DEFAULT_JOBS = max(1, min(len(UNITS), os.cpu_count() or 1))
+
+
+def build(jobs: int = DEFAULT_JOBS, verbose: bool = False, defines=(), out: str = LIB) -> str:
     """Compile every unit and link `out`. `defines` (NAME=VALUE) select build variants."""
     objdir = OBJ if not defines else OBJ + "_" + "_".join(d.replace("=", "") for d in defines)
     os.makedirs(objdir, exist_ok=True)
@@ -108,7 +112,7 @@ def build_host(verbose: bool = False) -> tuple[str, str]:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--jobs", type=int, default=3)
+    ap.add_argument("--jobs", type=int, default=DEFAULT_JOBS)
     ap.add_argument("-v", "--verbose", action="store_true")
     ap.add_argument("--define", action="append", default=[], help="NAME=VALUE for a build variant")
     ap.add_argument("--out", default=LIB)

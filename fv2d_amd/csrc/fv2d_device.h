@@ -259,30 +259,23 @@ auto with_class(const StageClass &c, F &&f) {
 #undef FV2D_CLS
 }
 
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW>
+// Single-grid launch chain: run-time switches -> template arguments. DIVF is fixed by the
+// caller (stage_fp64.hip / stage_fp64_exact.hip build one division form each, so the two
+// forms compile in separate units).
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int NW, int DIVF>
 hipError_t launch_stage_n(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
-  if constexpr (sizeof(R) == 8) {
-    if (kp.divx == DIV_EXACT && kp.divy == DIV_EXACT) {
-      typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW, DIVF_EXACT> K;
-      hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
-    } else {
-      typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW, DIVF_GENERAL> K;
-      hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
-    }
-  } else {
-    typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW> K;
-    hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
-  }
+  typedef StripStage<R, S, RECON, RIEM, TC, VISC, XG, NW, DIVF> K;
+  hipLaunchKernelGGL((strip_kernel<K, R, S>), dim3(g.nblocks), dim3(K::NT), 0, st, kp, io, g);
   return hipGetLastError();
 }
 
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG>
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, bool XG, int DIVF>
 hipError_t launch_stage_x(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
   switch (g.nw) {
   case 4:
-    return launch_stage_n<R, S, RECON, RIEM, TC, VISC, XG, 4>(kp, io, g, st);
+    return launch_stage_n<R, S, RECON, RIEM, TC, VISC, XG, 4, DIVF>(kp, io, g, st);
   case 8:
-    return launch_stage_n<R, S, RECON, RIEM, TC, VISC, XG, 8>(kp, io, g, st);
+    return launch_stage_n<R, S, RECON, RIEM, TC, VISC, XG, 8, DIVF>(kp, io, g, st);
   default:
     return hipErrorInvalidValue;
   }
@@ -291,45 +284,45 @@ hipError_t launch_stage_x(const KParams &kp, const StageIO<S> &io, const StripGe
 // Gravity, the well-balanced wall flux and wall conduction modes need the XG variant.
 inline bool needs_xg(const KParams &kp) { return kp.grav != GRAV_NONE || kp.wb || kp.bctc_ymin || kp.bctc_ymax; }
 
-template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC>
+template <typename R, typename S, int RECON, int RIEM, bool TC, bool VISC, int DIVF>
 hipError_t launch_stage_t(const KParams &kp, const StageIO<S> &io, const StripGeom &g, hipStream_t st) {
-  if (needs_xg(kp)) return launch_stage_x<R, S, RECON, RIEM, TC, VISC, true>(kp, io, g, st);
-  return launch_stage_x<R, S, RECON, RIEM, TC, VISC, false>(kp, io, g, st);
+  if (needs_xg(kp)) return launch_stage_x<R, S, RECON, RIEM, TC, VISC, true, DIVF>(kp, io, g, st);
+  return launch_stage_x<R, S, RECON, RIEM, TC, VISC, false, DIVF>(kp, io, g, st);
 }
 
-template <typename R, typename S, int RECON, int RIEM>
+template <typename R, typename S, int RECON, int RIEM, int DIVF>
 hipError_t launch_stage_rr(const KParams &kp, const StageIO<S> &io, const StripGeom &g, bool tc, bool visc,
                            hipStream_t st) {
   if (tc) {
-    if (visc) return launch_stage_t<R, S, RECON, RIEM, true, true>(kp, io, g, st);
-    return launch_stage_t<R, S, RECON, RIEM, true, false>(kp, io, g, st);
+    if (visc) return launch_stage_t<R, S, RECON, RIEM, true, true, DIVF>(kp, io, g, st);
+    return launch_stage_t<R, S, RECON, RIEM, true, false, DIVF>(kp, io, g, st);
   }
-  if (visc) return launch_stage_t<R, S, RECON, RIEM, false, true>(kp, io, g, st);
-  return launch_stage_t<R, S, RECON, RIEM, false, false>(kp, io, g, st);
+  if (visc) return launch_stage_t<R, S, RECON, RIEM, false, true, DIVF>(kp, io, g, st);
+  return launch_stage_t<R, S, RECON, RIEM, false, false, DIVF>(kp, io, g, st);
 }
 
-template <typename R, typename S, int RECON>
+template <typename R, typename S, int RECON, int DIVF>
 hipError_t launch_stage_r(const KParams &kp, const StageIO<S> &io, const StripGeom &g, int riem, bool tc, bool visc,
                           hipStream_t st) {
   switch (riem) {
   case HLL:
-    return launch_stage_rr<R, S, RECON, HLL>(kp, io, g, tc, visc, st);
+    return launch_stage_rr<R, S, RECON, HLL, DIVF>(kp, io, g, tc, visc, st);
   case FSLP:
-    return launch_stage_rr<R, S, RECON, FSLP>(kp, io, g, tc, visc, st);
+    return launch_stage_rr<R, S, RECON, FSLP, DIVF>(kp, io, g, tc, visc, st);
   default:
-    return launch_stage_rr<R, S, RECON, HLLC>(kp, io, g, tc, visc, st);
+    return launch_stage_rr<R, S, RECON, HLLC, DIVF>(kp, io, g, tc, visc, st);
   }
 }
 
 // PCM_WB computes a well-balanced face and then falls through to PCM (Update.h:27-33).
-template <typename R, typename S>
+template <typename R, typename S, int DIVF>
 hipError_t launch_stage(const KParams &kp, const StageIO<S> &io, const StripGeom &g, int recon, int riem, bool tc,
                         bool visc, hipStream_t st) {
-  if (recon == PLM) return launch_stage_r<R, S, PLM>(kp, io, g, riem, tc, visc, st);
-  return launch_stage_r<R, S, PCM>(kp, io, g, riem, tc, visc, st);
+  if (recon == PLM) return launch_stage_r<R, S, PLM, DIVF>(kp, io, g, riem, tc, visc, st);
+  return launch_stage_r<R, S, PCM, DIVF>(kp, io, g, riem, tc, visc, st);
 }
 
-// Defined in stage_fp64.hip / stage_fp32.hip.
+// Defined in stage_fp64.hip / stage_fp64_exact.hip / stage_fp32.hip.
 hipError_t launch_batch_fp64(const StageClass &c, const BatchJob *jobs, const BatchItem *items, const int *range,
                              int *counter, int stage, int grid, hipStream_t st);
 hipError_t launch_batch_fp32(const StageClass &c, const BatchJob *jobs, const BatchItem *items, const int *range,
@@ -338,6 +331,8 @@ int batch_occupancy_fp64(const StageClass &c);
 int batch_occupancy_fp32(const StageClass &c);
 hipError_t launch_stage_fp64(const KParams &kp, const StageIO<double> &io, const StripGeom &g, int recon, int riem,
                              bool tc, bool visc, hipStream_t st);
+hipError_t launch_stage_fp64_exact(const KParams &kp, const StageIO<double> &io, const StripGeom &g, int recon,
+                                   int riem, bool tc, bool visc, hipStream_t st);
 hipError_t launch_stage_fp32(const KParams &kp, const StageIO<float> &io, const StripGeom &g, int recon, int riem,
                              bool tc, bool visc, hipStream_t st);
 hipError_t launch_stage_fp32s64(const KParams &kp, const StageIO<double> &io, const StripGeom &g, int recon,

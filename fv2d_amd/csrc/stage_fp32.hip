@@ -7,13 +7,13 @@ namespace fv2d {
 
 hipError_t launch_stage_fp32(const KParams &kp, const StageIO<float> &io, const StripGeom &g, int recon, int riem,
                              bool tc, bool visc, hipStream_t st) {
-  return launch_stage<float, float>(kp, io, g, recon, riem, tc, visc, st);
+  return launch_stage<float, float, DIVF_RUNTIME>(kp, io, g, recon, riem, tc, visc, st);
 }
 
 // FP32 arithmetic on a state stored in FP64 (FV2D_FP32_STATE64).
 hipError_t launch_stage_fp32s64(const KParams &kp, const StageIO<double> &io, const StripGeom &g, int recon,
                                 int riem, bool tc, bool visc, hipStream_t st) {
-  return launch_stage<float, double>(kp, io, g, recon, riem, tc, visc, st);
+  return launch_stage<float, double, DIVF_RUNTIME>(kp, io, g, recon, riem, tc, visc, st);
 }
 
 namespace {

@@ -1,4 +1,6 @@
-// stage_fp64.hip -- FP64 (parity form) instantiations of the stage kernel.
+// stage_fp64.hip -- FP64 (parity form) instantiations of the stage kernel: the single-grid
+// kernels with the general division form and the batch kernels; the exact-division
+// single-grid kernels are in stage_fp64_exact.hip (compiled in parallel).
 // Built with -ffp-contract=off: every a*b+c stays a separate multiply and add, as in the
 // reference's CPU build, so results match it bit for bit.
 #include "fv2d_device.h"
@@ -7,7 +9,9 @@ namespace fv2d {
 
 hipError_t launch_stage_fp64(const KParams &kp, const StageIO<double> &io, const StripGeom &g, int recon, int riem,
                              bool tc, bool visc, hipStream_t st) {
-  return launch_stage<double, double>(kp, io, g, recon, riem, tc, visc, st);
+  if (kp.divx == DIV_EXACT && kp.divy == DIV_EXACT)
+    return launch_stage_fp64_exact(kp, io, g, recon, riem, tc, visc, st);
+  return launch_stage<double, double, DIVF_GENERAL>(kp, io, g, recon, riem, tc, visc, st);
 }
 
 namespace {
