@@ -239,3 +239,46 @@ def test_banded_schedule_general_division(Nx, Ny, wall, oracle_lib, gpu_lib):
                         bctc_ymin_value=1.5, bctc_ymax_value=0.3)
     assert run_pair(p, P.random_state(p, seed=Nx + Ny), 3, oracle_lib, gpu_lib)
 
+
+def extreme_state(p, seed):
+    """A smooth random state with patches whose values leave the exact ranges of the fast
+    division and square-root sequences: cold gas (p = 1e-250: squared sound speeds below
+    2^-767), p = 0, tiny densities (1e-300), subnormal velocities, and very large density
+    and pressure (1e150)."""
+    q = P.random_state(p, seed=seed)
+    Ny, Nx = q.shape[1:]
+    w = max(Nx // 8, 2)
+    patches = [(1.0, 0.1, 0.0, 1e-250), (1.0, 0.0, 0.0, 0.0), (1e-300, 1.0, 1e-10, 1e-300),
+               (1.0, 1e-315, 1e-320, 1.0), (1e150, 1e-70, 0.0, 1e150)]
+    for k, vals in enumerate(patches):
+        i0 = (k * Nx) // len(patches)
+        for v, x in enumerate(vals):
+            q[v, Ny // 4:Ny // 4 + Ny // 2, i0:i0 + w] = x
+    return q
+
+
+@pytest.mark.parametrize("riem,domain", [(HLLC, 1.0), (HLLC, 1.3), (HLL, 1.0), (FSLP, 1.3)])
+def test_extreme_states_take_exact_fallbacks(riem, domain, oracle_lib, gpu_lib):
+    """Values outside the exact ranges of the fast FP64 sequences, inside a stage: every lane
+    that leaves them must get the IEEE quotient and square root, so dt, reset and NaN counts
+    and every field (NaN positions included) equal the oracle's, bitwise; power-of-two and
+    general cell sizes."""
+    p = Params.defaults(Nx=64, Ny=40, xmin=0.0, xmax=domain, ymin=0.0, ymax=domain * 40 / 64, gamma0=1.4,
+                        CFL=0.4, reconstruction=PLM, riemann_solver=riem, tc_active=1, kappa=1e-3, visc_active=1,
+                        mu=1e-3, boundary_x=BC_PERIODIC, boundary_y=BC_REFLECTING,
+                        gravity_mode=GRAV_CONSTANT if riem == FSLP else GRAV_NONE, gy=-0.5)
+    q0 = extreme_state(p, seed=7 + riem)
+    o, g = Oracle(p, oracle_lib), Solver(p, gpu_lib)
+    o.set_prim(q0)
+    g.set_prim(q0)
+    for n in range(3):
+        dto, _ = o.compute_dt()
+        dtg, _ = g.compute_dt()
+        assert dtg == dto or (dtg != dtg and dto != dto), f"step {n}: dt {dtg!r} != {dto!r}"
+        co, cg = o.step(dto), g.step(dtg)
+        assert cg == co, f"step {n}: counts {cg} != {co}"
+        qo, qg = o.get_prim(), g.get_prim()
+        assert np.array_equal(qg, qo, equal_nan=True), f"step {n}: fields differ"
+    g.close()
+    o.close()
+
