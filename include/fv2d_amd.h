@@ -100,7 +100,10 @@ typedef struct fv2d_params {
   double epsilon_reset_negative; /* value written over negative density / pressure */
   /* this build */
   int32_t precision; /* fv2d_precision */
-  int32_t seg_rows;   /* rows per workgroup segment of the stage kernel, >= 4 (0: automatic) */
+  int32_t seg_rows;   /* rows per workgroup segment of the stage kernel, >= 4 (0: automatic); on
+                         grids of 2048 or more segments these are the long segments, and the
+                         last seg_rows rows of each eighth of the grid run in segments of a
+                         quarter of it (the launch then ends on short workgroups) */
   int32_t chunk_rows; /* rows per chunk = waves per workgroup of the stage kernel: 4 or 8 (0: automatic) */
   int32_t reserved[5];
 } fv2d_params;
