@@ -34,6 +34,12 @@ inline int check_params(const fv2d_params *p, std::string &why) {
     why = "grid larger than 2^30 cells";
     return FV2D_E_INVALID;
   }
+  // the stage kernel addresses a row's cells with 32-bit byte offsets (URow, fv2d_stage.h)
+  if (p->Nx > (1 << 26)) {
+    snprintf(b, sizeof b, "Nx=%d is wider than 2^26 cells", p->Nx);
+    why = b;
+    return FV2D_E_INVALID;
+  }
   if (!(p->xmax > p->xmin) || !(p->ymax > p->ymin)) {
     why = "empty domain (xmax <= xmin or ymax <= ymin)";
     return FV2D_E_INVALID;

@@ -639,7 +639,7 @@ struct StripStage {
 
   // Accumulator of the update. FP64: U itself, every term added in the reference's order
   // (Update.h:158-166, States.h:49-101). FP32 form: the increment dt*dF of the step in
-  // FP32, added to U in FP64 once at the end.
+  // FP32, added to U once at the end (in FP32 for the FP32 state, in FP64 for the FP64 one).
   typedef typename std::conditional<sizeof(R) == 8, double, float>::type A;
 
   // U += ((FL - FR) * dt) / d (Update.h:158-159); FP32 form: (FL - FR) * (dt / d).

@@ -54,10 +54,11 @@ enum fv2d_tc_mode { FV2D_TCM_CONSTANT = 0, FV2D_TCM_B02 = 1 };
 enum fv2d_bctc_mode { FV2D_BCTC_NONE = 0, FV2D_BCTC_FIXED_TEMPERATURE = 1, FV2D_BCTC_FIXED_GRADIENT = 2 };
 /* Arithmetic form. FP64 is the parity form. FP32 runs reconstruction, Riemann fluxes and the
  * parabolic/gravity stencils in FP32 and stores the state in FP32; the step's increment dt*dF
- * is formed in FP32 and added to U in FP64 (with the RK2 average), and the conservation sums
- * and the dt reduction are FP64. FP32_STATE64 is the FP32 form with the state U stored in
- * FP64 (the increment is added to the FP64 state, which is never rounded to FP32): twice the
- * state bytes of FP32, the FP32 form's arithmetic. Batches take FP64 or FP32. */
+ * is formed in FP32 and added to U, with the RK2 average, in FP32 (each rounded once, like
+ * the storage); the dt reduction over cells and the conservation sums are FP64.
+ * FP32_STATE64 is the FP32 form with the state U stored in FP64 (the increment is added to
+ * the FP64 state in FP64, and the state is never rounded to FP32): twice the state bytes of
+ * FP32, the FP32 form's arithmetic. Batches take FP64 or FP32. */
 enum fv2d_precision { FV2D_FP64 = 0, FV2D_FP32 = 1, FV2D_FP32_STATE64 = 2 };
 
 /*
@@ -68,7 +69,7 @@ enum fv2d_precision { FV2D_FP64 = 0, FV2D_FP32 = 1, FV2D_FP32_STATE64 = 2 };
  */
 typedef struct fv2d_params {
   /* [mesh] */
-  int32_t Nx, Ny;   /* domain cells */
+  int32_t Nx, Ny;   /* domain cells: 2 <= Nx <= 2^26, Ny >= 2, Nx * Ny <= 2^30 */
   int32_t Ng;       /* ghost layers (>= 2 for PLM); results do not depend on it */
   double xmin, xmax, ymin, ymax;
   /* [solvers] */

@@ -43,6 +43,8 @@ def test_abi_version_and_defaults():
 @pytest.mark.parametrize("kw,code,frag", [
     (dict(tc_active=1, tc_mode=1), capi.FV2D_E_UNSUPPORTED, "B02"),
     (dict(Nx=1), capi.FV2D_E_INVALID, "Nx and Ny"),
+    (dict(Nx=(1 << 26) + 1, Ny=2), capi.FV2D_E_INVALID, "wider than 2^26"),
+    (dict(Nx=1 << 16, Ny=(1 << 14) + 1), capi.FV2D_E_INVALID, "2^30 cells"),
     (dict(reconstruction=2, Ng=1), capi.FV2D_E_INVALID, "Nghosts"),
     (dict(xmax=-1.0), capi.FV2D_E_INVALID, "empty domain"),
     (dict(riemann_solver=7), capi.FV2D_E_INVALID, "out of range"),
