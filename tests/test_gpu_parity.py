@@ -282,3 +282,10 @@ def test_extreme_states_take_exact_fallbacks(riem, domain, oracle_lib, gpu_lib):
     g.close()
     o.close()
 
+
+def test_banded_schedule_chunk_rows_8(oracle_lib, gpu_lib):
+    """8-row chunks on a grid large enough for the banded schedule (KH 2048^2: long segments
+    of 4 chunks, short ones of 2), 4 steps, every field compared."""
+    p, q0, _ = P.workload("kh", N=2048)
+    assert run_pair(p.copy(chunk_rows=8), q0, 4, oracle_lib, gpu_lib, every=2)
+
