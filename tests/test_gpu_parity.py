@@ -240,6 +240,20 @@ def test_banded_schedule_general_division(Nx, Ny, wall, oracle_lib, gpu_lib):
     assert run_pair(p, P.random_state(p, seed=Nx + Ny), 3, oracle_lib, gpu_lib)
 
 
+@pytest.mark.parametrize("Nx,Ny,recon,riem,ts", [(2048, 1100, PCM_WB, HLL, 0), (1100, 2300, PLM, FSLP, 1)])
+def test_banded_schedule_analytical_gravity(Nx, Ny, recon, riem, ts, oracle_lib, gpu_lib):
+    """Banded-schedule grids with the branches the test above leaves out: analytical
+    (hot-bubble) gravity on a domain with ymin != 0, the well-balanced PCM reconstruction
+    or the FSLP solver, HLL, absorbing x walls, Euler or RK2 stepping, and the reversed wall
+    conduction modes; 3 steps, every step compared."""
+    p = Params.defaults(Nx=Nx, Ny=Ny, xmin=0.0, xmax=2.0, ymin=-1.0, ymax=2.0, gamma0=1.4, CFL=0.4,
+                        reconstruction=recon, riemann_solver=riem, time_stepping=ts, tc_active=1, kappa=3e-4,
+                        visc_active=1, mu=2e-4, boundary_x=BC_ABSORBING, boundary_y=BC_REFLECTING,
+                        gravity_mode=GRAV_ANALYTICAL, hot_bubble_g0=0.3, well_balanced_flux_at_y_bc=1,
+                        bctc_ymin=2, bctc_ymax=1, bctc_ymin_value=0.3, bctc_ymax_value=1.5)
+    assert run_pair(p, P.random_state(p, seed=Nx + 3 * Ny), 3, oracle_lib, gpu_lib)
+
+
 def extreme_state(p, seed):
     """A smooth random state with patches whose values leave the exact ranges of the fast
     division and square-root sequences: cold gas (p = 1e-250: squared sound speeds below
