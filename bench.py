@@ -10,8 +10,11 @@ synthetic seeded initial state, all state resident in HBM. One step = one full R
 (dt reduction + two fused stage launches). The FP64 parity form gives `value`
 (Gcell-updates/s = Nx*Ny*K / time of K steps, HIP events on the solver stream, device
 synchronised on both sides); the FP32 form is timed the same way and reported beside it.
-The K steps run as one device loop (fv2d_advance); `repetitions` gives the median, minimum
-and maximum rate of --reps further loops of K steps each (when K >= 16). `pcie_inclusive`
+The K steps run as one device loop (fv2d_advance), after W untimed warm-up steps; before
+those, each form runs its untimed 3000-step KH drift measurement (the GPU clocks rise over
+the first ~50-100 ms of sustained load, so the timed loop starts at the running clock).
+`repetitions` gives the median, minimum and maximum rate of --reps further loops of K steps
+each (when K >= 16). `pcie_inclusive`
 adds the upload of the host primitives before and their read-back after (host clock).
 
 roofline: the dominant kernel is the fused stage kernel (two launches per step). Its
@@ -73,7 +76,10 @@ def read_traffic(prec: str, size: int):
     return float(m.group(1)) if m else None
 
 
-def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps: int = 5):
+def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps: int = 5, before=None):
+    """One form on KH size^2: upload q0, run `before` (untimed GPU work of the bench's own, see
+    below), `warmup` untimed steps, then the K = `steps` timed steps as one device loop, then
+    `reps` further loops of K steps for the spread."""
     from fv2d_amd import hiprt
     from fv2d_amd import problems as P
     from fv2d_amd.capi import FP32, FP32_STATE64, FP64
@@ -86,11 +92,17 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps:
     s.set_prim(q0)  # host primitives over PCIe, then U = primToCons on the device
     hiprt.synchronize()
     t_up = time.perf_counter() - t_up
+    # The GPU raises its clocks over the first ~50-100 ms of sustained load (a K = 20 loop
+    # right after 5 warm-up steps reads ~7% slow in FP64, ~20% in FP32, a second loop close
+    # to the steady rate; profiling on or off makes no difference). The accuracy run of this
+    # form goes here, after the upload, so the warm-up and timed steps start at the running clock.
+    before_out = before() if before is not None else None
     if warmup:
         s.advance(max_steps=warmup, record_dts=False)
     stream = s.lib.fv2d_get_stream(s.h)
     hiprt.synchronize()
-    s.lib.fv2d_profile(s.h, 1)
+    # per-launch events for the K steps' stage launches, created here, before the timed loop
+    assert s.lib.fv2d_profile(s.h, STAGE_LAUNCHES_PER_STEP * steps) == 0
     # the K timed steps: one device loop, HIP events on the solver stream around it
     ev = [hiprt.Event() for _ in range(2)]
     ev[0].record(stream)
@@ -129,7 +141,7 @@ def run_gpu(prec: str, size: int, steps: int, warmup: int, q0: np.ndarray, reps:
     if rep_rates:
         reps_out.update(gcells_median=round(float(np.median(rep_rates)), 4), gcells_min=round(min(rep_rates), 4),
                         gcells_max=round(max(rep_rates), 4))
-    return dict(gcells=gcells, ms_per_step=ms / steps, mean_launch_ms=mean_launch_s * 1e3, launches=int(st_n.value),
+    return dict(before=before_out, gcells=gcells, ms_per_step=ms / steps, mean_launch_ms=mean_launch_s * 1e3, launches=int(st_n.value),
                 achieved=achieved, bytes_per_launch=bpl * cells, counts=counts, q=q, sums=sums,
                 pcie={"upload_ms": round(t_up * 1e3, 2), "download_ms": round(t_down * 1e3, 2),
                       "gcells": round(cells * steps / (ms * 1e-3 + t_up + t_down) / 1e9, 4)},
@@ -342,23 +354,23 @@ def errors_vs_oracle_all_configs():
     return out
 
 
-def kh_long_run_drift(size: int = 4096, steps: int = 3000):
+KH_DRIFT_STEPS = 3000
+
+
+def kh_drift(prec: str, size: int = 4096, steps: int = KH_DRIFT_STEPS):
     """Relative drift of the conserved totals (mass, x / y momentum, energy; momenta relative
-    to the mass) over `steps` KH steps, for the FP64 form and both FP32 forms."""
+    to the mass) over `steps` KH steps of one form ("fp64", "fp32", "fp32s64")."""
     from fv2d_amd.capi import FP32, FP32_STATE64, FP64, PROB_KH
     from fv2d_amd import problems as P
     from fv2d_amd.solver import Solver
 
-    out = {}
-    for name, prec in (("fp64", FP64), ("fp32", FP32), ("fp32s64", FP32_STATE64)):
-        with Solver(P.kh_params(N=size, precision=prec)) as s:
-            s.set_problem(PROB_KH, seed=12345, amplitude=0.01)
-            s0 = s.sums()
-            s.advance(max_steps=steps, record_dts=False)
-            s1 = s.sums()
-        out[name] = {k: float(abs(s1[i] - s0[i]) / max(abs(s0[i]), abs(s0[0])))
-                     for i, k in enumerate(("mass", "x_momentum", "y_momentum", "energy"))}
-    return {"steps": steps, "grid": [size, size], "rel_drift": out}
+    with Solver(P.kh_params(N=size, precision={"fp64": FP64, "fp32": FP32, "fp32s64": FP32_STATE64}[prec])) as s:
+        s.set_problem(PROB_KH, seed=12345, amplitude=0.01)
+        s0 = s.sums()
+        s.advance(max_steps=steps, record_dts=False)
+        s1 = s.sums()
+    return {k: float(abs(s1[i] - s0[i]) / max(abs(s0[i]), abs(s0[0])))
+            for i, k in enumerate(("mass", "x_momentum", "y_momentum", "energy"))}
 
 
 def calibrate_copy(mib: int, reps: int = 4) -> int:
@@ -435,21 +447,29 @@ def main(argv=None):
     log(f"bench: building KH {a.size}^2 initial state")
     q0 = P.kh_ic(P.kh_params(N=a.size))
 
+    fp32_err = drift = None
+    accuracy = not a.no_fp32 and not a.no_errors and a.size == 4096
+
+    def drift_run(prec):
+        if not accuracy:
+            return None
+        return lambda: kh_drift(prec, a.size)
+
     log(f"bench: FP64 form, {a.warmup} warmup + {a.steps} timed steps")
-    r64 = run_gpu("fp64", a.size, a.steps, a.warmup, q0, a.reps)
+    r64 = run_gpu("fp64", a.size, a.steps, a.warmup, q0, a.reps, before=drift_run("fp64"))
     log(f"bench: FP64 {r64['gcells']:.3f} Gcell/s, {r64['ms_per_step']:.3f} ms/step")
     r32 = None
     if not a.no_fp32:
         log(f"bench: FP32 form, {a.warmup} warmup + {a.steps} timed steps")
-        r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0, a.reps)
+        r32 = run_gpu("fp32", a.size, a.steps, a.warmup, q0, a.reps, before=drift_run("fp32"))
         log(f"bench: FP32 {r32['gcells']:.3f} Gcell/s, {r32['ms_per_step']:.3f} ms/step")
         log(f"bench: FP32 form with the state in FP64, {a.warmup} warmup + {a.steps} timed steps")
-        r32s = run_gpu("fp32s64", a.size, a.steps, a.warmup, q0, a.reps)
+        r32s = run_gpu("fp32s64", a.size, a.steps, a.warmup, q0, a.reps, before=drift_run("fp32s64"))
         log(f"bench: FP32 (state FP64) {r32s['gcells']:.3f} Gcell/s, {r32s['ms_per_step']:.3f} ms/step")
-    fp32_err = drift = None
-    if not a.no_fp32 and not a.no_errors and a.size == 4096:
-        drift = kh_long_run_drift(a.size)
-        log(f"bench: KH 3000-step mass drift: {drift['rel_drift']}")
+    if accuracy:
+        drift = {"steps": KH_DRIFT_STEPS, "grid": [a.size, a.size],
+                 "rel_drift": {"fp64": r64["before"], "fp32": r32["before"], "fp32s64": r32s["before"]}}
+        log(f"bench: KH {KH_DRIFT_STEPS}-step drift: {drift['rel_drift']}")
         fp32_err = errors_vs_oracle_all_configs()
     serving = None
     if not a.no_serve:

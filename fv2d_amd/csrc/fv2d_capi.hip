@@ -73,12 +73,15 @@ static bool is_fp32(const fv2d_handle *h) { return sim_fp32(h->d); }
     if (_e != hipSuccess) FAILH(h, FV2D_E_HIP, "%s: %s", #call, hipGetErrorString(_e));                              \
   } while (0)
 
-// fv2d_profile: an event on the handle's stream before and after every stage launch.
+// fv2d_profile: an event on the handle's stream before and after every stage launch. The
+// events only time the launches (they are read after a stream synchronization), so they skip
+// the system-scope memory fence a default event performs when it completes.
+static const unsigned kProfEventFlags = hipEventDisableSystemFence;
 static hipError_t prof_mark(void *ctx) {
   fv2d_handle *h = (fv2d_handle *)ctx;
   if (h->ev_used == h->ev.size()) {
     hipEvent_t e;
-    hipError_t r = hipEventCreate(&e);
+    hipError_t r = hipEventCreateWithFlags(&e, kProfEventFlags);
     if (r != hipSuccess) return r;
     h->ev.push_back(e);
   }
@@ -421,8 +424,17 @@ void *fv2d_get_stream(fv2d_handle *h) { return h ? (void *)h->d.stream : nullptr
 
 int fv2d_profile(fv2d_handle *h, int enable) {
   if (!h) return FV2D_E_INVALID;
+  if (enable < 0) FAILH(h, FV2D_E_INVALID, "fv2d_profile: enable=%d must be >= 0", enable);
   HIPH(h, hipSetDevice(h->device));
   HIPH(h, hipStreamSynchronize(h->d.stream));
+  // the events of the launches to come are created here, not while they are enqueued: a
+  // hipEventCreate inside a device loop holds the host back and starves the GPU
+  const size_t want = enable > 1 ? 2 * (size_t)enable : (enable ? 256 : 0);
+  while (h->ev.size() < want) {
+    hipEvent_t e;
+    HIPH(h, hipEventCreateWithFlags(&e, kProfEventFlags));
+    h->ev.push_back(e);
+  }
   h->prof = enable != 0;
   h->d.mark = h->prof ? prof_mark : nullptr;
   h->d.mark_ctx = h;

@@ -188,7 +188,10 @@ void *fv2d_get_stream(fv2d_handle *h);
 /* Kernel timing, the counterpart of the reference's Kokkos kernel labels used by Kokkos
  * Tools (e.g. "Update", Update.h:102). While enabled, every stage-kernel launch is bracketed
  * by HIP events on the handle's stream; fv2d_profile_read returns the summed duration (ms)
- * and the number of timed stage launches since the last read, then resets them. */
+ * and the number of timed stage launches since the last read, then resets them. enable: 0
+ * off; 1 on, with events for 128 launches created before returning; n > 1 on, with events
+ * for n launches created before returning (more are created as the launches need them, which
+ * costs host time inside the caller's timed region). FV2D_E_INVALID for enable < 0. */
 int fv2d_profile(fv2d_handle *h, int enable);
 int fv2d_profile_read(fv2d_handle *h, double *stage_ms, int64_t *stage_launches);
 

@@ -88,3 +88,26 @@ def test_unsupported_option_is_rejected_at_create(gpu_lib):
     with pytest.raises(Fv2dError) as e:
         Solver(Params.defaults(tc_active=1, tc_mode=capi.TCM_B02), gpu_lib)
     assert e.value.code == capi.FV2D_E_UNSUPPORTED and "B02" in str(e.value)
+
+
+@pytest.mark.parametrize("enable", [1, 6, 40])
+def test_stage_profiling_counts(enable, gpu_lib):
+    """fv2d_profile: every stage launch of an RK2 step (2 per step) is timed, whether the events
+    were created up front (enable = launch count) or some are created as the launches need
+    them (enable = 1 or fewer than the launches); a read resets the count; enable < 0 is
+    refused and leaves profiling as it was."""
+    p = Params.defaults(Nx=64, Ny=32, time_stepping=1)
+    with Solver(p, gpu_lib) as s:
+        s.set_problem(capi.PROB_SOD_X)
+        assert gpu_lib.fv2d_profile(s.h, -1) == capi.FV2D_E_INVALID
+        assert b"enable" in gpu_lib.fv2d_last_error(s.h)
+        assert gpu_lib.fv2d_profile(s.h, enable) == 0
+        ms, n = C.c_double(), C.c_int64()
+        for steps in (10, 3):
+            s.advance(max_steps=steps, record_dts=False)
+            assert gpu_lib.fv2d_profile_read(s.h, C.byref(ms), C.byref(n)) == 0
+            assert n.value == 2 * steps and ms.value > 0.0
+        assert gpu_lib.fv2d_profile(s.h, 0) == 0
+        s.advance(max_steps=2, record_dts=False)
+        assert gpu_lib.fv2d_profile_read(s.h, C.byref(ms), C.byref(n)) == 0
+        assert n.value == 0 and ms.value == 0.0
