@@ -12,7 +12,7 @@ import pytest
 
 import emu as EMU
 from fv2d_amd import problems as P
-from fv2d_amd.capi import FP32, PLM, Params
+from fv2d_amd.capi import BC_PERIODIC, BC_REFLECTING, FP32, HLLC, PLM, Params
 from oracle.oracle import Oracle
 
 COMBOS = list(itertools.product([0, 1, 2], [0, 1, 2], [0, 1, 2], [0, 1, 2], [0, 1], [0, 1], [0, 1], [0, 1, 2], [0, 1]))
@@ -106,6 +106,33 @@ def test_banded_schedule_bitwise(Nx, Ny, seg, nw, emu_lib, oracle_lib):
     for n, c in enumerate([(1, 1, PLM, 1, 1, 1, 1, 1, 1), (2, 0, PLM, 0, 0, 0, 1, 0, 0), (0, 2, 0, 2, 1, 1, 0, 2, 1)]):
         p, q0 = make(n, *c, Nx=Nx, Ny=Ny)
         compare(p.copy(chunk_rows=nw), q0, emu_lib, oracle_lib, steps=2, banded=seg)
+
+
+@pytest.mark.parametrize("nw", [4, 8])
+def test_reset_and_nan_counts(nw, emu_lib, oracle_lib):
+    """Negative-pressure resets (a strong rarefaction) and NaN fields (a NaN patch spreading
+    through the stencil) counted alike, 4- and 8-row chunks, several strips and segments."""
+    p = Params.defaults(Nx=130, Ny=40, boundary_x=BC_REFLECTING, boundary_y=BC_PERIODIC, gamma0=1.4, CFL=0.9,
+                        riemann_solver=HLLC, reconstruction=PLM, chunk_rows=nw, seg_rows=12)
+    q0 = np.zeros((4, 40, 130))
+    q0[0] = 1.0
+    q0[3] = 1e-6
+    q0[1, :, :65] = -0.5
+    q0[1, :, 65:] = 0.5
+    q0[2, 20:23, 100:103] = np.nan
+    o, e = Oracle(p, oracle_lib), EMU.Emu(p, emu_lib)
+    o.set_prim(q0)
+    e.set_prim(q0)
+    seen = [0, 0, 0]
+    for n in range(4):
+        dto, _ = o.compute_dt()
+        dte, _ = e.compute_dt()
+        assert dte == dto or (dte != dte and dto != dto)
+        co, ce = o.step(dto), e.step(dte)
+        assert ce == co, f"step {n}: counts {ce} != {co}"
+        seen = [a + b for a, b in zip(seen, co)]
+        assert np.array_equal(o.get_prim(), e.get_prim(), equal_nan=True)
+    assert min(seen) > 0, f"not every kind of reset was triggered: {seen}"
 
 
 def test_sanitized_bounds(oracle_lib):
