@@ -29,29 +29,15 @@ __device__ inline double wave_max(double m) {
   return m;
 }
 
-// The resets of one row of cell updates (LaneRegs::flags) into the wave's totals: one
-// wave-wide test, the per-bit counts only when some lane reset something.
-template <typename Regs>
-__device__ inline void count_resets(Regs &r) {
-  if (__ballot(r.flags != 0u) == 0ull) return;
-  r.neg_rho += (unsigned)__popcll(__ballot(r.flags & 1u));
-  r.neg_prs += (unsigned)__popcll(__ballot(r.flags & 2u));
-  r.nan_cnt += (unsigned)(__popcll(__ballot(r.flags & 4u)) + 2 * __popcll(__ballot(r.flags & 8u)) +
-                          4 * __popcll(__ballot(r.flags & 16u)));
-}
-
-// Epilogue commit: reset counters (rare, atomics; nr, np, nn are the wave's totals) and the
-// workgroup's max inverse dt into partials[blk] (no atomics on the hot value: the reduction
-// kernel reads all partials).
+// Epilogue commit: reset counters (rare, atomics) and the workgroup's max inverse dt into
+// partials[blk] (no atomics on the hot value: the reduction kernel reads all partials).
 template <int NT>
 __device__ inline void commit_epilogue(unsigned nr, unsigned np, unsigned nn, double inv, StepCtl *ctl,
                                        double *partials, int blk) {
   __shared__ double red[NT / 64];
-  if ((threadIdx.x & 63) == 0) {
-    if (nr) atomicAdd(&ctl->neg_rho, (unsigned long long)nr);
-    if (np) atomicAdd(&ctl->neg_prs, (unsigned long long)np);
-    if (nn) atomicAdd(&ctl->nan_cnt, (unsigned long long)nn);
-  }
+  if (nr) atomicAdd(&ctl->neg_rho, (unsigned long long)nr);
+  if (np) atomicAdd(&ctl->neg_prs, (unsigned long long)np);
+  if (nn) atomicAdd(&ctl->nan_cnt, (unsigned long long)nn);
   const double m = wave_max(inv);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -120,7 +106,6 @@ __device__ inline void strip_rows(const KParams &kp, const StageIO<S> &io, int i
     }
     __syncthreads();
     K::update_cell(lane, w, i0, jb, jc, je, sm, kp, io, dt, dt_dx, dt_dy, r);
-    count_resets(r);
     __syncthreads();
     xw = (xw + 1) % K::NW;
   }

@@ -251,11 +251,7 @@ struct LaneRegs {
   R py;          // its interface pressure
   R tcx, tcxl;   // conduction flux at x faces (l, l+1) and (l-1, l)
   R vx[3], vxl[3];  // viscous x-face terms mu*tau_xx, mu*tau_xy, mu*(tau.u) at (l,l+1), (l-1,l)
-  // Reset counts of the workgroup's rows so far. On the device each wave keeps its own totals
-  // (the same in every lane: scalar registers), added from `flags` after every cell update.
   unsigned neg_rho, neg_prs, nan_cnt;
-  // The last cell update's resets: bit 0 density, bit 1 pressure, bits 2-4 the NaN fields.
-  unsigned flags;
   double inv;
 };
 
@@ -680,7 +676,6 @@ struct StripStage {
                                     double dt_dy, Regs &r) {
     const int divx = EXD ? int(DIV_EXACT) : kp.divx, divy = EXD ? int(DIV_EXACT) : kp.divy;
     const int j = jc + w, gi = i0 - 2 + lane;
-    r.flags = 0;
     if (lane < 2 || gi >= kp.Nx || j >= je) return;
     const R rho = Q(sm, IR, j, lane, jb);
 
@@ -820,18 +815,16 @@ struct StripStage {
       const DivBy<R> byr(un.v[IR], un.v[IU]);
       St<R> q = cons_to_prim(un, R(kp.gm1), byr);
       const R eps = R(kp.eps_reset);
-      unsigned f = 0;
       if (q.v[IR] < R(0)) {
         q.v[IR] = eps;
-        f |= 1u;
+        r.neg_rho += 1;
       }
       if (q.v[IP] < R(0)) {
         q.v[IP] = eps;
-        f |= 2u;
+        r.neg_prs += 1;
       }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) f += (q.v[v] != q.v[v]) ? 4u : 0u;
-      r.flags = f;
+      for (int v = 0; v < 4; ++v) r.nan_cnt += (q.v[v] != q.v[v]) ? 1u : 0u;
       // speedOfSound (States.h:45-46): (p * gamma) / rho, by the same divisor unless reset
       R c2;
       if (q.v[IR] == un.v[IR])

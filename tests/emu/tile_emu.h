@@ -107,8 +107,6 @@ void run_stage(Sim &s, const StageIO<S> &io) {
           for (int v = 0; v < 3; ++v) d.vxl[v] = s_ ? s_->vx[v] : R(0);
       });
       LANES(K::update_cell(lane, w, i0, jb, jc, je, *sm, kp, io, s.ctl.dt, s.ctl.dt_dx, s.ctl.dt_dy, r));
-      // the kernel's count_resets, lane by lane (the totals are summed over lanes below)
-      LANES(r.neg_rho += r.flags & 1u; r.neg_prs += (r.flags >> 1) & 1u; r.nan_cnt += r.flags >> 2);
       xw = (xw + 1) % K::NW;
     }
 #undef LANES
