@@ -507,6 +507,8 @@ def main(argv=None):
             "boundaries": "periodic",
         },
         "repetitions": r64["blocks"],
+        "before_warmup": ("the form's untimed KH 3000-step drift run (the GPU at its running clock when the timed "
+                          "loop starts)" if accuracy else "nothing (the timed loop may start at a ramping GPU clock)"),
         "pcie_inclusive": dict(r64["pcie"], note="the same K steps with the host primitives uploaded before "
                                "(fv2d_set_prim) and read back after (fv2d_get_prim), host clock; not `value`"),
         "hbm_frac_step": round(BYTES_PER_CELL_STEP["fp64"] * cells / (r64["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
